@@ -1,0 +1,268 @@
+/*
+ * np_sssp.h -- C ABI of the MI355X (gfx950) SSSP / recursive-BFS nested-parallelism path.
+ *
+ * Plain pointers and sizes only.  Every entry point returns an int status (NP_OK or a
+ * negative NP_ERR_* code) and records a message readable with np_last_error() (per
+ * host thread).  The library owns one HIP stream per context; a context is not
+ * reentrant, separate contexts are.
+ *
+ * Graph layout is the reference CSR of common/global.h:28-42 as produced by
+ * convertCSR() (common/global.cpp:192-235): row[n+1] (vertexArray, row[n] = m),
+ * col[m] (edgeArray), w[m] (weightArray), all int32.  Distances are int32 with
+ * unreached = NP_INF (= INF, common/global.h:19); BFS levels use NP_UNDEFINED
+ * (apps/bfs-rec/src/gpu/bfs_rec.h:13).
+ *
+ * Reference interfaces replaced (the reference has no C ABI: its host code calls the
+ * CUDA wrappers directly, so these stand where those calls were):
+ *   np_graph_upload          prepare_gpu()'s cudaMalloc + H2D of the CSR
+ *                            (apps/sssp/src/gpu/sssp_wrapper.cu:58-140)
+ *   np_sssp                  sssp_np_consolidate_gpu() + the D2H of costArray
+ *                            (sssp_wrapper.cu:475-546, :576), i.e. SSSP_GPU() with
+ *                            config.solution == 6 (sssp_wrapper.cu:548-583)
+ *   np_bfs                   bfs_rec_dp_cons_gpu() + D2H of levelArray
+ *                            (apps/bfs-rec/src/gpu/bfs_rec_wrapper.cu:170-229, :264)
+ *   np_sched.level           the CONSOLIDATE_LEVEL macro (sssp_wrapper.cu:14-16,503-531;
+ *                            bfs_rec_wrapper.cu:20-22,180-214), now a runtime value
+ *   np_sched.log2_t_lo       THRESHOLD (apps/sssp/src/gpu/sssp_kernel.cu:7), as a power of two
+ *   np_graph_free/np_destroy clean_gpu() (sssp_wrapper.cu:142-149)
+ *   np_read_graph            readInputDIMACS9/DIMACS10/SLNDC + convertCSR
+ *                            (common/global.cpp:29-235)
+ * Entry points without a counterpart in the reference (batching, the learned
+ * scheduler, the generators, the autotuner) say so in their comment.
+ */
+#ifndef NP_SSSP_H
+#define NP_SSSP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NP_INF 1073741824      /* common/global.h:19 */
+#define NP_UNDEFINED 0x7FFFFFFF /* apps/bfs-rec/src/gpu/bfs_rec.h:13 */
+
+enum {
+    NP_OK = 0,
+    NP_ERR_ARG = -1,           /* bad argument (NULL, source out of range, bad sizes) */
+    NP_ERR_HIP = -2,           /* a HIP runtime call failed */
+    NP_ERR_NOMEM = -3,         /* device or host allocation failed */
+    NP_ERR_CAPACITY = -4,      /* a size limit of the path was exceeded */
+    NP_ERR_NOT_CONVERGED = -5, /* iteration cap reached before the frontier emptied */
+    NP_ERR_TIMEOUT = -6,       /* a grid barrier's bounded spin expired (kernel exited) */
+    NP_ERR_UNSUPPORTED = -7,   /* no usable gfx950 device / feature */
+    NP_ERR_IO = -8             /* file could not be read or is malformed */
+};
+
+/* Consolidation level (CONSOLIDATE_LEVEL: 0 warp, 1 block, 2 grid). */
+enum { NP_LEVEL_WAVE = 0, NP_LEVEL_WORKGROUP = 1, NP_LEVEL_GRID = 2, NP_LEVEL_LEARNED = -1 };
+/* How the iteration loop is driven. */
+enum {
+    NP_DRIVE_PERSISTENT = 0, /* one cooperative launch runs every iteration (grid barriers) */
+    NP_DRIVE_HOSTLOOP = 1,   /* kernels per iteration, 4-byte frontier readback per iteration
+                                (the reference's loop shape, sssp_wrapper.cu:490-541) */
+    NP_DRIVE_GRAPH = 2       /* iterations captured in a hipGraph, replayed in chunks */
+};
+/* How the next frontier is formed. */
+enum {
+    NP_FRONTIER_SCAN = 0, /* update bitmap compacted after each expansion
+                             (unorder_generateQueue_kernel, sssp_kernel.cu:648-658) */
+    NP_FRONTIER_PUSH = 1  /* relaxers append improved vertices directly (no O(n) scan) */
+};
+
+/* Schedule of one solve.  Bins by out-degree d of a frontier vertex:
+ *   d <  2^log2_t_lo                        lane bin: wavefront-wide balanced gather
+ *   2^log2_t_lo <= d < 2^log2_t_hi          wavefront bin: one 64-lane wavefront per vertex
+ *   d >= 2^log2_t_hi (and >= 2^log2_t_lo)   level 0: wavefront; level 1: one workgroup per
+ *                                           vertex; level 2: grid-wide, split in edge chunks
+ * The reference's fixed heuristic is log2_t_lo = log2_t_hi = 6 (THRESHOLD 64). */
+typedef struct np_sched {
+    int32_t level;       /* NP_LEVEL_*; NP_LEVEL_LEARNED = the trained MLP picks per iteration */
+    int32_t log2_t_lo;   /* 0..30 */
+    int32_t log2_t_hi;   /* 0..30 */
+    int32_t drive;       /* NP_DRIVE_* */
+    int32_t frontier;    /* NP_FRONTIER_* */
+    int32_t wgs_per_cu;  /* persistent grid: workgroups per CU (0 = occupancy maximum) */
+    int32_t graph_chunk; /* NP_DRIVE_GRAPH: iterations per captured chunk (0 = 16) */
+    int32_t max_iters;   /* 0 = no cap beyond n+1 */
+    int32_t trace;       /* nonzero: record per-iteration trace (np_trace_*) */
+    int32_t reserved[7];
+} np_sched;
+
+/* Fills the plain fixed-threshold schedule of the reference at `level` (0,1,2), the
+ * persistent drive and the scan frontier. */
+void np_sched_default(int32_t level, np_sched *s);
+
+typedef struct np_stats {
+    int32_t iterations;     /* frontier expansions performed */
+    int32_t status;         /* same as the return code */
+    double kernel_ms;       /* device time of the solve, hipEvents on the context stream */
+    int64_t frontier_total; /* sum of frontier sizes over iterations */
+    int64_t reserved[6];
+} np_stats;
+
+typedef struct np_ctx np_ctx;
+typedef struct np_graph np_graph;
+typedef struct np_batch np_batch;
+
+const char *np_last_error(void);
+const char *np_version(void);
+
+/* Context on HIP device `device` (cudaSetDevice in prepare_gpu, sssp_wrapper.cu:67). */
+int np_create(int device, np_ctx **out);
+/* Make the library launch on `stream` (a hipStream_t) instead of its own. */
+int np_set_stream(np_ctx *ctx, void *stream);
+void *np_get_stream(np_ctx *ctx);
+int np_destroy(np_ctx *ctx);
+/* Number of compute units and the persistent-kernel occupancy (diagnostics). */
+int np_device_info(np_ctx *ctx, int32_t *num_cus, int32_t *max_wgs_per_cu);
+
+/* Device-memory plumbing for callers without their own HIP runtime (no counterpart in
+ * the reference; its wrapper calls cudaMalloc/cudaMemcpy directly). */
+int np_dev_alloc(np_ctx *ctx, uint64_t bytes, void **d_ptr);
+int np_dev_free(np_ctx *ctx, void *d_ptr);
+/* kind 0: host -> device, 1: device -> host, 2: device -> device; synchronous on the
+ * context stream. */
+int np_dev_copy(np_ctx *ctx, void *dst, const void *src, uint64_t bytes, int32_t kind);
+/* Waits for all work queued on the context stream. */
+int np_sync(np_ctx *ctx);
+
+/* Copy a host CSR to the device (prepare_gpu's H2D, sssp_wrapper.cu:100-113). */
+int np_graph_upload(np_ctx *ctx, int32_t n, int64_t m, const int32_t *row, const int32_t *col,
+                    const int32_t *w, np_graph **out);
+/* Use device-resident CSR arrays in place (no copy; the caller keeps them alive).
+ * No counterpart in the reference (it always re-uploads). */
+int np_graph_wrap_device(np_ctx *ctx, int32_t n, int64_t m, const int32_t *d_row,
+                         const int32_t *d_col, const int32_t *d_w, np_graph **out);
+int np_graph_info(const np_graph *g, int32_t *n, int64_t *m);
+int np_graph_download(np_ctx *ctx, const np_graph *g, int32_t *row, int32_t *col, int32_t *w);
+/* Device addresses of the CSR arrays (row, col, w). */
+int np_graph_device_ptrs(const np_graph *g, const int32_t **row, const int32_t **col,
+                         const int32_t **w);
+int np_graph_free(np_graph *g);
+
+/* Single-source shortest paths (SSSP_GPU with solution 6).  dist_out: host int32[n]. */
+int np_sssp(np_ctx *ctx, np_graph *g, int32_t src, const np_sched *sched, int32_t *dist_out,
+            np_stats *stats);
+/* Same, distances written to device memory d_dist (int32[n]); nothing is copied to the
+ * host and the call returns once the solve has finished on the device. */
+int np_sssp_device(np_ctx *ctx, np_graph *g, int32_t src, const np_sched *sched, int32_t *d_dist,
+                   np_stats *stats);
+/* Recursive BFS levels (BFS_REC_GPU with solution 3): unit weights, level[src] = 0,
+ * unreached = NP_UNDEFINED.  level_out: host int32[n]. */
+int np_bfs(np_ctx *ctx, np_graph *g, int32_t src, const np_sched *sched, int32_t *level_out,
+           np_stats *stats);
+
+/* Batched solves -- no counterpart in the reference.  `count` instances; instance i
+ * solves graph graphs[i] from source srcs[i] (graphs may repeat: many sources on one
+ * graph; or differ: many graphs of uneven size).  All instances run in one persistent
+ * launch (or one captured graph) over a shared, instance-tagged worklist.  unit != 0
+ * solves BFS levels instead of distances. */
+int np_batch_create(np_ctx *ctx, int32_t count, np_graph *const *graphs, const int32_t *srcs,
+                    int32_t unit, np_batch **out);
+int np_batch_solve(np_ctx *ctx, np_batch *b, const np_sched *sched, np_stats *stats);
+/* Host copy of instance i's distances (int32[n_i]). */
+int np_batch_get(np_ctx *ctx, np_batch *b, int32_t i, int32_t *dist_out);
+/* Device address of instance i's distances. */
+int np_batch_device_dist(np_batch *b, int32_t i, const int32_t **d_dist);
+int np_batch_free(np_batch *b);
+
+/* Work counted for throughput: n_r = vertices with a finite distance, m_r = sum of
+ * their out-degrees (device reduction over d_dist, which holds inf for unreached). */
+int np_reached_work(np_ctx *ctx, np_graph *g, const int32_t *d_dist, int32_t inf,
+                    int64_t *n_r, int64_t *m_r);
+
+/* ---- per-iteration trace (sched.trace != 0) -- no counterpart in the reference ---- */
+#define NP_NCLASS 32 /* out-degree classes: 0 for d == 0, else 1 + floor(log2 d) */
+typedef struct np_trace_rec {
+    int32_t iteration;
+    int32_t level;            /* configuration used for this iteration */
+    int32_t log2_t_lo, log2_t_hi;
+    int32_t count[NP_NCLASS]; /* frontier size per out-degree class */
+    int64_t t_begin, t_end;   /* device clock (100 MHz) at the start/end of the expansion */
+} np_trace_rec;
+/* Copies up to `cap` records of the last solve on ctx; *count = records available. */
+int np_trace_get(np_ctx *ctx, np_trace_rec *recs, int32_t cap, int32_t *count);
+
+/* ---- learned scheduler -- no counterpart in the reference --------------------------
+ * fp32 MLP 10 -> 32 -> 32 -> 5 (3 level logits, 2 regressions: log2_t_lo, log2_t_hi),
+ * ReLU hidden layers; parameters W1[32][10] b1[32] W2[32][32] b2[32] W3[5][32] b3[5]
+ * (NP_MLP_NPARAM floats).  Features per iteration (np_features documents them). */
+#define NP_MLP_NIN 10
+#define NP_MLP_NPARAM (32 * 10 + 32 + 32 * 32 + 32 + 5 * 32 + 5)
+int np_model_set(np_ctx *ctx, const float *params); /* host params -> device */
+int np_model_get(np_ctx *ctx, float *params);
+/* Feature vector of one frontier (class counts) at iteration `it` of a graph with n
+ * vertices (the same function the kernels evaluate, for host-side training sets). */
+int np_features(const int32_t *count, int32_t it, int32_t n, float *x);
+/* Fixed-threshold heuristic applied when the network is absent or its output is not
+ * finite: the fallback and the judge of the learned scheduler. */
+int np_heuristic(const int32_t *count, int32_t it, int32_t n, np_sched *s);
+/* One full-batch training run on the GPU with the project's forward/backward kernels
+ * (Adam).  x[batch][10], label[batch] in {0,1,2}, target[batch][2] (log2 thresholds).
+ * Updates the context's model in place; writes the loss per epoch into loss_out. */
+int np_model_train(np_ctx *ctx, const float *x, const int32_t *label, const float *target,
+                   int32_t batch, int32_t epochs, float lr, float lam, float *loss_out);
+/* Loss and gradient (host outputs) of the current model on a batch -- for gradient checks. */
+int np_model_loss_grad(np_ctx *ctx, const float *x, const int32_t *label, const float *target,
+                       int32_t batch, float lam, float *loss, float *grad);
+/* Forward pass on the GPU: out[batch][5]. */
+int np_model_forward(np_ctx *ctx, const float *x, int32_t batch, float *out);
+
+/* ---- single-iteration stepping, for timing labels -- no counterpart in the reference --
+ * Starts a stepped solve; each np_step_run performs one expansion with the given
+ * schedule (level/thresholds) from a saved state, returning its device time, so that
+ * every candidate schedule is timed on the same frontier. */
+int np_step_begin(np_ctx *ctx, np_graph *g, int32_t src, np_stats *stats);
+/* Frontier class counts of the current state; returns NP_OK and *done = 1 when empty. */
+int np_step_frontier(np_ctx *ctx, int32_t *count, int32_t *done);
+/* Time one expansion with `sched` from the current state without advancing it. */
+int np_step_try(np_ctx *ctx, const np_sched *sched, int32_t reps, double *ms);
+/* Advance the state by one expansion with `sched`. */
+int np_step_advance(np_ctx *ctx, const np_sched *sched, double *ms);
+int np_step_end(np_ctx *ctx, int32_t *dist_out);
+
+/* ---- graph generators (device) -- no counterpart in the reference ------------------
+ * Deterministic from `seed` (SplitMix64-keyed counters).  CSR per-vertex order is the
+ * reverse of generation order, as convertCSR produces for a file read in that order. */
+int np_gen_rmat(np_ctx *ctx, int32_t scale, int32_t edge_factor, double a, double b, double c,
+                int32_t wmin, int32_t wmax, uint64_t seed, int32_t permute, np_graph **out);
+int np_gen_er(np_ctx *ctx, int32_t n, int64_t m, int32_t wmin, int32_t wmax, uint64_t seed,
+              np_graph **out);
+int np_gen_grid(np_ctx *ctx, int32_t rows, int32_t cols, int64_t n_long, int32_t wmin,
+                int32_t wmax, uint64_t seed, np_graph **out);
+/* Host versions of the same generators (identical CSR), into malloc'd arrays freed with
+ * np_host_free. */
+int np_gen_rmat_host(int32_t scale, int32_t edge_factor, double a, double b, double c,
+                     int32_t wmin, int32_t wmax, uint64_t seed, int32_t permute, int32_t **row,
+                     int32_t **col, int32_t **w, int32_t *n, int64_t *m);
+int np_gen_er_host(int32_t n, int64_t m, int32_t wmin, int32_t wmax, uint64_t seed, int32_t **row,
+                   int32_t **col, int32_t **w);
+int np_gen_grid_host(int32_t rows, int32_t cols, int64_t n_long, int32_t wmin, int32_t wmax,
+                     uint64_t seed, int32_t **row, int32_t **col, int32_t **w, int64_t *m);
+/* `k` distinct sources with out-degree >= 1 drawn with `seed` (Graph500 practice). */
+int np_pick_sources(const int32_t *row, int32_t n, int32_t k, uint64_t seed, int32_t *srcs);
+void np_host_free(void *p);
+
+/* ---- graph files: readInputDIMACS9/10/SLNDC + convertCSR (common/global.cpp:29-235) --
+ * fmt 0 DIMACS9, 1 DIMACS10, 2 SLNDC (apps/sssp/src/gpu/sssp.cpp:120-125).  Same CSR as the
+ * reference (per-vertex order reversed, DIMACS10/SLNDC weights rand()%1000+1 after
+ * srand(7) in file order), built without adjacency lists.  *src gets the DIMACS9 's'
+ * line (1-based in the file) or 0.  Arrays are freed with np_host_free. */
+int np_read_graph(int32_t fmt, const char *path, int32_t *n, int64_t *m, int32_t *src,
+                  int32_t **row, int32_t **col, int32_t **w);
+/* Same, from an open FILE* (passed as void*), as the reference readers use the global fp. */
+int np_read_graph_stream(int32_t fmt, void *fp, int32_t *n, int64_t *m, int32_t *src,
+                         int32_t **row, int32_t **col, int32_t **w);
+
+/* ---- autotuner -- no counterpart in the reference ----------------------------------
+ * Times the candidate schedules (level x log2_t_lo x log2_t_hi x wgs_per_cu) on `g`
+ * from the given sources and returns the fastest in *best (plain drive/frontier kept
+ * from `base`).  ms_out (optional, cap entries) receives each candidate's mean time. */
+int np_autotune(np_ctx *ctx, np_graph *g, const int32_t *srcs, int32_t nsrc, const np_sched *base,
+                np_sched *best, double *best_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NP_SSSP_H */

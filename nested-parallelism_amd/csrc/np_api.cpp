@@ -1,0 +1,1081 @@
+// np_api.cpp -- the C ABI (include/np_sssp.h) over the gfx950 kernels.
+//
+// Reference mapping: np_graph_upload is prepare_gpu()'s allocation + H2D
+// (apps/sssp/src/gpu/sssp_wrapper.cu:58-140); np_sssp is sssp_np_consolidate_gpu()
+// (:475-546) followed by the D2H of costArray (:576); np_bfs is bfs_rec_dp_cons_gpu()
+// (apps/bfs-rec/src/gpu/bfs_rec_wrapper.cu:170-229).  Unlike the reference, the device
+// graph stays resident between solves and the solver state is allocated once per graph.
+#include <hip/hip_runtime.h>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <cmath>
+#include <vector>
+#include <algorithm>
+#include <string>
+
+#include "np_internal.h"
+
+namespace np {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+}
+
+static int hip_fail(hipError_t e, const char *what, int line) {
+    set_error("%s failed (line %d): %s", what, line, hipGetErrorString(e));
+    return NP_ERR_HIP;
+}
+#define HIPCHK(x)                                                      \
+    do {                                                               \
+        hipError_t _e = (x);                                           \
+        if (_e != hipSuccess) return hip_fail(_e, #x, __LINE__);       \
+    } while (0)
+
+template <class T>
+static hipError_t dalloc(T **p, size_t count) {
+    *p = nullptr;
+    return hipMalloc((void **)p, sizeof(T) * (count ? count : 1));
+}
+template <class T>
+static void dfree(T *&p) {
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+static constexpr int TRACE_CAP = 1 << 15;
+
+}  // namespace np
+
+using namespace np;
+
+// ------------------------------------------------------------------------------------
+// solver state shared by single-graph solves and batches
+// ------------------------------------------------------------------------------------
+struct Space {
+    long long nslots = 0, nwords = 0;
+    long long qcap = 0;
+    int hist[NCLS] = {0};
+    int *dist = nullptr;          // owned distance buffer (np_sssp copies out of it)
+    unsigned *bm[2] = {nullptr, nullptr};
+    int *queue[2] = {nullptr, nullptr};
+    int *cnt = nullptr;
+    int *seg_off = nullptr;
+    unsigned char *cls = nullptr; // owned for batches; borrowed from the graph otherwise
+    bool own_cls = false;
+    int *lut = nullptr;
+    int2 *inst = nullptr;
+    int *src_slots = nullptr;
+    int src_cap = 0;
+    // captured chunk of the graph drive
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    SchedDev gsp{};
+    DevState gst{};
+    int gchunk = 0, gwgs = 0;
+    bool gbatch = false, gunit = false;
+
+    void release() {
+        dfree(dist); dfree(bm[0]); dfree(bm[1]); dfree(queue[0]); dfree(queue[1]);
+        dfree(cnt); dfree(seg_off);
+        if (own_cls) dfree(cls);
+        cls = nullptr;
+        dfree(lut); dfree(inst); dfree(src_slots);
+        drop_graph();
+    }
+    void drop_graph() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        gexec = nullptr; graph = nullptr;
+    }
+};
+
+struct np_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int num_cus = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    Ctrl *ctrl = nullptr;
+    Ctrl *h_ctrl = nullptr;          // pinned readback
+    np_trace_rec *trace = nullptr;
+    int last_iters = 0;
+    float *model = nullptr;          // NP_MLP_NPARAM floats on the device
+    bool has_model = false;
+    float *adam_m = nullptr, *adam_v = nullptr;
+    int adam_step = 0;
+    unsigned long long *d_work2 = nullptr;
+    // stepping
+    np_graph *step_g = nullptr;
+    int step_it = 0;
+    int *snap_dist = nullptr;
+    unsigned *snap_bm = nullptr;
+    long long snap_n = 0;
+};
+
+struct np_graph {
+    np_ctx *ctx = nullptr;
+    int n = 0;
+    long long m = 0;
+    int *row = nullptr, *col = nullptr, *w = nullptr;
+    bool owns = false;
+    unsigned char *cls = nullptr;
+    int hist[NCLS] = {0};
+    Space sp;  // single-source solver state (lazily allocated)
+    bool sp_ready = false;
+};
+
+struct np_batch {
+    np_ctx *ctx = nullptr;
+    int count = 0;
+    int unit = 0;
+    std::vector<int> base, nverts;
+    int *row = nullptr, *col = nullptr, *w = nullptr;  // concatenated graphs (or borrowed)
+    bool owns_graph = false;
+    Space sp;
+};
+
+// ------------------------------------------------------------------------------------
+// helpers
+// ------------------------------------------------------------------------------------
+static int ctx_sync(np_ctx *c) {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return NP_OK;
+}
+
+static void seg_from_hist(const int *hist, int *seg, long long *qcap) {
+    long long acc = 0;
+    for (int c = 0; c < NCLS; ++c) {
+        seg[c] = (int)acc;
+        if (c > 0) acc += hist[c];  // class-0 vertices are never queued
+    }
+    seg[NCLS] = (int)acc;
+    *qcap = acc;
+}
+
+// Allocates the frontier/bitmap/counter buffers of a space whose hist/nslots are set.
+static int space_alloc(np_ctx *c, Space &s, bool want_dist) {
+    int seg[NCLS + 1];
+    seg_from_hist(s.hist, seg, &s.qcap);
+    s.nwords = (s.nslots + 31) / 32;
+    if (want_dist) HIPCHK(dalloc(&s.dist, (size_t)s.nslots));
+    HIPCHK(dalloc(&s.bm[0], (size_t)s.nwords));
+    HIPCHK(dalloc(&s.bm[1], (size_t)s.nwords));
+    HIPCHK(dalloc(&s.queue[0], (size_t)s.qcap));
+    HIPCHK(dalloc(&s.queue[1], (size_t)s.qcap));
+    HIPCHK(dalloc(&s.cnt, 3 * NCLS));
+    HIPCHK(dalloc(&s.seg_off, NCLS + 1));
+    HIPCHK(hipMemcpyAsync(s.seg_off, seg, sizeof(seg), hipMemcpyHostToDevice, c->stream));
+    return ctx_sync(c);
+}
+
+static int ensure_src(np_ctx *c, Space &s, int nsrc) {
+    if (s.src_cap >= nsrc) return NP_OK;
+    dfree(s.src_slots);
+    HIPCHK(dalloc(&s.src_slots, (size_t)nsrc));
+    s.src_cap = nsrc;
+    (void)c;
+    return NP_OK;
+}
+
+static DevState make_state(np_ctx *c, Space &s, const int *row, const int *col, const int *w,
+                           int *dist, int inf) {
+    DevState st{};
+    st.row = row; st.col = col; st.w = w;
+    st.cls = s.cls;
+    st.dist = dist;
+    st.bm[0] = s.bm[0]; st.bm[1] = s.bm[1];
+    st.queue[0] = s.queue[0]; st.queue[1] = s.queue[1];
+    st.seg_off = s.seg_off;
+    st.cnt = s.cnt;
+    st.lut = s.lut;
+    st.inst = s.inst;
+    st.nslots = s.nslots;
+    st.nwords = s.nwords;
+    st.inf = inf;
+    st.ctrl = c->ctrl;
+    return st;
+}
+
+static int validate_sched(const np_sched *in, np_sched *out) {
+    if (in) *out = *in;
+    else np_sched_default(1, out);
+    if (out->level < -1 || out->level > 2) { set_error("level must be -1..2"); return NP_ERR_ARG; }
+    if (out->log2_t_lo < 0 || out->log2_t_lo > 30 || out->log2_t_hi < 0 || out->log2_t_hi > 30) {
+        set_error("log2 thresholds must be 0..30"); return NP_ERR_ARG;
+    }
+    if (out->drive < 0 || out->drive > 2) { set_error("bad drive"); return NP_ERR_ARG; }
+    if (out->frontier < 0 || out->frontier > 1) { set_error("bad frontier mode"); return NP_ERR_ARG; }
+    if (out->wgs_per_cu < 0 || out->graph_chunk < 0 || out->max_iters < 0) {
+        set_error("negative schedule field"); return NP_ERR_ARG;
+    }
+    return NP_OK;
+}
+
+static SchedDev make_schedd(np_ctx *c, const np_sched &s, int n_feat) {
+    SchedDev d{};
+    d.learned = (s.level == NP_LEVEL_LEARNED) ? 1 : 0;
+    d.level = s.level >= 0 ? s.level : 1;  // learned: fallback is the plain block level
+    d.klo = s.log2_t_lo;
+    d.khi = s.log2_t_hi;
+    d.push = s.frontier == NP_FRONTIER_PUSH;
+    // With non-negative weights the frontier empties within n+1 expansions (every vertex
+    // is final once all shortest paths of <= k edges are), so n+2 only stops inputs
+    // with negative cycles.
+    d.max_iters = s.max_iters > 0 ? s.max_iters : (n_feat < 0x7FFFFFFD ? n_feat + 2 : 0x7FFFFFFF);
+    d.trace_cap = s.trace ? TRACE_CAP : 0;
+    d.trace = s.trace ? c->trace : nullptr;
+    d.n_feat = n_feat;
+    d.mlp = (d.learned && c->has_model) ? c->model : nullptr;
+    if (d.learned && !c->has_model) d.learned = 0;  // no network: the heuristic runs
+    return d;
+}
+
+static int persistent_wgs(np_ctx *c, const np_sched &s, bool batch, bool unit) {
+    int occ = persistent_max_wgs_per_cu(batch, unit, s.frontier == NP_FRONTIER_PUSH);
+    if (occ < 1) occ = 1;
+    int per = s.wgs_per_cu > 0 ? std::min(s.wgs_per_cu, occ) : occ;
+    return per * c->num_cus;
+}
+
+static int read_ctrl(np_ctx *c) {
+    HIPCHK(hipMemcpyAsync(c->h_ctrl, c->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, c->stream));
+    return ctx_sync(c);
+}
+
+// Runs a solve whose state was initialised; fills stats.
+static int run_drive(np_ctx *c, Space &s, const DevState &st, const np_sched &sch, const SchedDev &sd,
+                     bool batch, bool unit, np_stats *stats) {
+    int rc = NP_OK;
+    float ms = 0.f;
+    if (sch.drive == NP_DRIVE_PERSISTENT) {
+        int wgs = persistent_wgs(c, sch, batch, unit);
+        HIPCHK(hipEventRecord(c->ev0, c->stream));
+        HIPCHK(launch_persistent(st, sd, batch, unit, wgs, c->stream));
+        HIPCHK(hipEventRecord(c->ev1, c->stream));
+        if ((rc = read_ctrl(c))) return rc;
+    } else if (sch.drive == NP_DRIVE_HOSTLOOP) {
+        const int wgs = c->num_cus * 8;
+        HIPCHK(hipEventRecord(c->ev0, c->stream));
+        for (int it = 0;; ++it) {
+            if (!sd.push) HIPCHK(launch_compact(st, it, batch, wgs, c->stream));
+            HIPCHK(launch_decide(st, sd, it, c->stream));
+            HIPCHK(launch_expand(st, sd, it, batch, unit, wgs, c->stream));
+            // one small readback per iteration, as the reference's loop (sssp_wrapper.cu:539)
+            if ((rc = read_ctrl(c))) return rc;
+            if (c->h_ctrl->done || c->h_ctrl->error) break;
+        }
+        HIPCHK(hipEventRecord(c->ev1, c->stream));
+        if ((rc = read_ctrl(c))) return rc;
+    } else {
+        const int K = sch.graph_chunk > 0 ? sch.graph_chunk : 16;
+        const int wgs = c->num_cus * 8;
+        bool same = s.gexec && s.gchunk == K && s.gwgs == wgs && s.gbatch == batch && s.gunit == unit &&
+                    std::memcmp(&s.gsp, &sd, sizeof(SchedDev)) == 0 &&
+                    std::memcmp(&s.gst, &st, sizeof(DevState)) == 0;
+        if (!same) {
+            s.drop_graph();
+            hipStream_t cs = nullptr;
+            HIPCHK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+            hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+            if (e == hipSuccess) {
+                for (int j = 0; j < K && e == hipSuccess; ++j) {
+                    if (!sd.push) e = launch_compact(st, j, batch, wgs, cs);
+                    if (e == hipSuccess) e = launch_decide(st, sd, j, cs);
+                    if (e == hipSuccess) e = launch_expand(st, sd, j, batch, unit, wgs, cs);
+                }
+                if (e == hipSuccess) e = launch_advance(st, K, cs);
+                hipGraph_t g = nullptr;
+                hipError_t e2 = hipStreamEndCapture(cs, &g);
+                if (e == hipSuccess) e = e2;
+                if (e == hipSuccess) {
+                    s.graph = g;
+                    e = hipGraphInstantiate(&s.gexec, g, nullptr, nullptr, 0);
+                } else if (g) {
+                    (void)hipGraphDestroy(g);
+                }
+            }
+            (void)hipStreamDestroy(cs);
+            if (e != hipSuccess) { s.drop_graph(); return hip_fail(e, "graph capture", __LINE__); }
+            s.gchunk = K; s.gwgs = wgs; s.gbatch = batch; s.gunit = unit; s.gsp = sd; s.gst = st;
+        }
+        HIPCHK(hipEventRecord(c->ev0, c->stream));
+        for (;;) {
+            HIPCHK(hipGraphLaunch(s.gexec, c->stream));
+            if ((rc = read_ctrl(c))) return rc;
+            if (c->h_ctrl->done || c->h_ctrl->error) break;
+        }
+        HIPCHK(hipEventRecord(c->ev1, c->stream));
+        if ((rc = read_ctrl(c))) return rc;
+    }
+    HIPCHK(hipEventSynchronize(c->ev1));
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    const Ctrl &h = *c->h_ctrl;
+    c->last_iters = h.iters;
+    rc = h.error ? h.error : NP_OK;
+    if (rc == NP_ERR_TIMEOUT) set_error("a grid barrier timed out; the kernel exited early");
+    if (rc == NP_ERR_NOT_CONVERGED) set_error("iteration cap reached");
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        stats->iterations = h.iters;
+        stats->status = rc;
+        stats->kernel_ms = ms;
+        stats->frontier_total = h.frontier_total;
+    }
+    return rc;
+}
+
+static int graph_space(np_ctx *c, np_graph *g) {
+    if (g->sp_ready) return NP_OK;
+    Space &s = g->sp;
+    s.nslots = g->n;
+    std::memcpy(s.hist, g->hist, sizeof(s.hist));
+    s.cls = g->cls;
+    s.own_cls = false;
+    int rc = space_alloc(c, s, true);
+    if (rc) { s.release(); return rc; }
+    if ((rc = ensure_src(c, s, 1))) return rc;
+    g->sp_ready = true;
+    return NP_OK;
+}
+
+static int solve_single(np_ctx *c, np_graph *g, int src, const np_sched *sched, int *d_dist_user,
+                        int *h_dist, bool unit, np_stats *stats) {
+    if (!c || !g) { set_error("null context or graph"); return NP_ERR_ARG; }
+    if (g->ctx != c) { set_error("graph belongs to another context"); return NP_ERR_ARG; }
+    if (src < 0 || src >= g->n) { set_error("source %d out of range [0,%d)", src, g->n); return NP_ERR_ARG; }
+    np_sched sch;
+    int rc = validate_sched(sched, &sch);
+    if (rc) return rc;
+    if ((rc = graph_space(c, g))) return rc;
+    Space &s = g->sp;
+    int *dist = d_dist_user ? d_dist_user : s.dist;
+    const int inf = unit ? UNDEFINED : INF;
+    DevState st = make_state(c, s, g->row, g->col, unit ? nullptr : g->w, dist, inf);
+    SchedDev sd = make_schedd(c, sch, g->n);
+    HIPCHK(hipMemcpyAsync(s.src_slots, &src, sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(launch_init_solve(st, s.src_slots, 1, sd.push, c->stream));
+    rc = run_drive(c, s, st, sch, sd, false, unit, stats);
+    if (rc) return rc;
+    if (h_dist) {
+        HIPCHK(hipMemcpyAsync(h_dist, dist, sizeof(int) * (size_t)g->n, hipMemcpyDeviceToHost, c->stream));
+        if ((rc = ctx_sync(c))) return rc;
+    }
+    return NP_OK;
+}
+
+static int finish_graph(np_ctx *c, np_graph *g) {
+    int *hist = nullptr;
+    int bad = 0;
+    HIPCHK(dalloc(&g->cls, (size_t)g->n));
+    HIPCHK(dalloc(&hist, NCLS + 1));
+    hipError_t e = launch_fill_i32(hist, NCLS + 1, 0, c->stream);
+    if (e == hipSuccess) e = launch_validate_csr(g->row, g->col, g->n, g->m, hist + NCLS, c->stream);
+    if (e == hipSuccess) e = launch_degree_class(g->row, g->n, g->cls, hist, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(g->hist, hist, sizeof(g->hist), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(&bad, hist + NCLS, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    dfree(hist);
+    if (e != hipSuccess) return hip_fail(e, "degree classes", __LINE__);
+    if (bad) {
+        set_error("invalid CSR: %d violations of row[0]=0, row non-decreasing, row[n]=m, 0<=col<n", bad);
+        return NP_ERR_ARG;
+    }
+    return NP_OK;
+}
+
+static void init_model_params(float *p) {
+    // He-style deterministic initialisation from SplitMix64 draws.
+    uint64_t key = stream_key(12345, 99);
+    const int fan[3] = {NP_MLP_NIN, 32, 32};
+    const int sizes[3] = {32 * NP_MLP_NIN, 32 * 32, 5 * 32};
+    const int bias[3] = {32, 32, 5};
+    int off = 0, k = 0;
+    for (int l = 0; l < 3; ++l) {
+        float sc = std::sqrt(2.0f / (float)fan[l]);
+        for (int i = 0; i < sizes[l]; ++i, ++k) {
+            double u = (double)(draw(key, (uint64_t)k) >> 11) * (1.0 / 9007199254740992.0);
+            p[off++] = (float)((2.0 * u - 1.0) * sc);
+        }
+        for (int i = 0; i < bias[l]; ++i) p[off++] = 0.f;
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// C ABI
+// ------------------------------------------------------------------------------------
+extern "C" {
+
+const char *np_last_error(void) { return g_err; }
+const char *np_version(void) { return "np-sssp gfx950 1.0"; }
+
+int np_create(int device, np_ctx **out) {
+    if (!out) { set_error("null out"); return NP_ERR_ARG; }
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        set_error("no HIP device visible");
+        return NP_ERR_UNSUPPORTED;
+    }
+    if (device < 0 || device >= ndev) { set_error("device %d out of range", device); return NP_ERR_ARG; }
+    HIPCHK(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        set_error("device %d is %s; this library is built for gfx950 only", device, prop.gcnArchName);
+        return NP_ERR_UNSUPPORTED;
+    }
+    if (!prop.cooperativeLaunch) { set_error("cooperative launch unsupported"); return NP_ERR_UNSUPPORTED; }
+    np_ctx *c = new np_ctx();
+    c->device = device;
+    c->num_cus = prop.multiProcessorCount;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    c->own_stream = true;
+    if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+    if (e == hipSuccess) e = dalloc(&c->ctrl, 1);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&c->h_ctrl, sizeof(Ctrl), hipHostMallocDefault);
+    if (e == hipSuccess) e = dalloc(&c->trace, TRACE_CAP);
+    if (e == hipSuccess) e = dalloc(&c->model, NP_MLP_NPARAM);
+    if (e == hipSuccess) e = dalloc(&c->d_work2, 4);
+    if (e != hipSuccess) {
+        int rc = hip_fail(e, "context setup", __LINE__);
+        np_destroy(c);
+        return rc;
+    }
+    std::memset(c->h_ctrl, 0, sizeof(Ctrl));
+    *out = c;
+    return NP_OK;
+}
+
+int np_set_stream(np_ctx *c, void *stream) {
+    if (!c || !stream) { set_error("null argument"); return NP_ERR_ARG; }
+    HIPCHK(hipSetDevice(c->device));
+    if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+    c->stream = (hipStream_t)stream;
+    c->own_stream = false;
+    return NP_OK;
+}
+
+void *np_get_stream(np_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+int np_destroy(np_ctx *c) {
+    if (!c) return NP_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    dfree(c->ctrl); dfree(c->trace); dfree(c->model); dfree(c->adam_m); dfree(c->adam_v);
+    dfree(c->d_work2); dfree(c->snap_dist); dfree(c->snap_bm);
+    if (c->h_ctrl) (void)hipHostFree(c->h_ctrl);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return NP_OK;
+}
+
+int np_device_info(np_ctx *c, int32_t *num_cus, int32_t *max_wgs_per_cu) {
+    if (!c) { set_error("null context"); return NP_ERR_ARG; }
+    if (num_cus) *num_cus = c->num_cus;
+    if (max_wgs_per_cu) *max_wgs_per_cu = persistent_max_wgs_per_cu(false, false, false);
+    return NP_OK;
+}
+
+int np_dev_alloc(np_ctx *c, uint64_t bytes, void **d_ptr) {
+    if (!c || !d_ptr) { set_error("null argument"); return NP_ERR_ARG; }
+    *d_ptr = nullptr;
+    HIPCHK(hipSetDevice(c->device));
+    hipError_t e = hipMalloc(d_ptr, bytes ? bytes : 1);
+    if (e != hipSuccess) { set_error("hipMalloc(%llu): %s", (unsigned long long)bytes, hipGetErrorString(e)); return NP_ERR_NOMEM; }
+    return NP_OK;
+}
+
+int np_dev_free(np_ctx *c, void *d_ptr) {
+    if (!c) { set_error("null context"); return NP_ERR_ARG; }
+    if (d_ptr) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipFree(d_ptr));
+    }
+    return NP_OK;
+}
+
+int np_dev_copy(np_ctx *c, void *dst, const void *src, uint64_t bytes, int32_t kind) {
+    if (!c || (!dst && bytes) || (!src && bytes)) { set_error("null argument"); return NP_ERR_ARG; }
+    if (kind < 0 || kind > 2) { set_error("bad copy kind"); return NP_ERR_ARG; }
+    if (!bytes) return NP_OK;
+    const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, k, c->stream));
+    return ctx_sync(c);
+}
+
+int np_sync(np_ctx *c) {
+    if (!c) { set_error("null context"); return NP_ERR_ARG; }
+    return ctx_sync(c);
+}
+
+static int new_graph(np_ctx *c, int32_t n, int64_t m, np_graph **out) {
+    if (!c || !out) { set_error("null argument"); return NP_ERR_ARG; }
+    if (n < 0 || m < 0) { set_error("negative sizes"); return NP_ERR_ARG; }
+    if (m > 0x7FFFFFFFll) { set_error("m = %lld exceeds the int32 CSR of the reference", (long long)m); return NP_ERR_CAPACITY; }
+    *out = nullptr;
+    return NP_OK;
+}
+
+int np_graph_upload(np_ctx *c, int32_t n, int64_t m, const int32_t *row, const int32_t *col,
+                    const int32_t *w, np_graph **out) {
+    int rc = new_graph(c, n, m, out);
+    if (rc) return rc;
+    if (!row || (m > 0 && (!col || !w))) { set_error("null CSR array"); return NP_ERR_ARG; }
+    if (row[0] != 0 || row[n] != m) { set_error("row[0] must be 0 and row[n] must be m"); return NP_ERR_ARG; }
+    HIPCHK(hipSetDevice(c->device));
+    np_graph *g = new np_graph();
+    g->ctx = c; g->n = n; g->m = m; g->owns = true;
+    hipError_t e = dalloc(&g->row, (size_t)n + 1);
+    if (e == hipSuccess) e = dalloc(&g->col, (size_t)m);
+    if (e == hipSuccess) e = dalloc(&g->w, (size_t)m);
+    if (e == hipSuccess) e = hipMemcpyAsync(g->row, row, sizeof(int) * ((size_t)n + 1), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess && m) e = hipMemcpyAsync(g->col, col, sizeof(int) * (size_t)m, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess && m) e = hipMemcpyAsync(g->w, w, sizeof(int) * (size_t)m, hipMemcpyHostToDevice, c->stream);
+    if (e != hipSuccess) { rc = hip_fail(e, "graph upload", __LINE__); np_graph_free(g); return rc; }
+    if ((rc = finish_graph(c, g))) { np_graph_free(g); return rc; }
+    *out = g;
+    return NP_OK;
+}
+
+int np_graph_wrap_device(np_ctx *c, int32_t n, int64_t m, const int32_t *d_row, const int32_t *d_col,
+                         const int32_t *d_w, np_graph **out) {
+    int rc = new_graph(c, n, m, out);
+    if (rc) return rc;
+    if (!d_row || (m > 0 && (!d_col || !d_w))) { set_error("null CSR array"); return NP_ERR_ARG; }
+    np_graph *g = new np_graph();
+    g->ctx = c; g->n = n; g->m = m; g->owns = false;
+    g->row = const_cast<int *>(d_row); g->col = const_cast<int *>(d_col); g->w = const_cast<int *>(d_w);
+    if ((rc = finish_graph(c, g))) { np_graph_free(g); return rc; }
+    *out = g;
+    return NP_OK;
+}
+
+int np_graph_info(const np_graph *g, int32_t *n, int64_t *m) {
+    if (!g) { set_error("null graph"); return NP_ERR_ARG; }
+    if (n) *n = g->n;
+    if (m) *m = g->m;
+    return NP_OK;
+}
+
+int np_graph_download(np_ctx *c, const np_graph *g, int32_t *row, int32_t *col, int32_t *w) {
+    if (!c || !g) { set_error("null argument"); return NP_ERR_ARG; }
+    if (row) HIPCHK(hipMemcpyAsync(row, g->row, sizeof(int) * ((size_t)g->n + 1), hipMemcpyDeviceToHost, c->stream));
+    if (col && g->m) HIPCHK(hipMemcpyAsync(col, g->col, sizeof(int) * (size_t)g->m, hipMemcpyDeviceToHost, c->stream));
+    if (w && g->m) HIPCHK(hipMemcpyAsync(w, g->w, sizeof(int) * (size_t)g->m, hipMemcpyDeviceToHost, c->stream));
+    return ctx_sync(c);
+}
+
+int np_graph_device_ptrs(const np_graph *g, const int32_t **row, const int32_t **col, const int32_t **w) {
+    if (!g) { set_error("null graph"); return NP_ERR_ARG; }
+    if (row) *row = g->row;
+    if (col) *col = g->col;
+    if (w) *w = g->w;
+    return NP_OK;
+}
+
+int np_graph_free(np_graph *g) {
+    if (!g) return NP_OK;
+    if (g->ctx) (void)hipSetDevice(g->ctx->device);
+    if (g->ctx && g->ctx->step_g == g) g->ctx->step_g = nullptr;
+    g->sp.release();
+    if (g->owns) { dfree(g->row); dfree(g->col); dfree(g->w); }
+    dfree(g->cls);
+    delete g;
+    return NP_OK;
+}
+
+int np_sssp(np_ctx *c, np_graph *g, int32_t src, const np_sched *sched, int32_t *dist_out, np_stats *stats) {
+    if (!dist_out) { set_error("null dist_out"); return NP_ERR_ARG; }
+    return solve_single(c, g, src, sched, nullptr, dist_out, false, stats);
+}
+
+int np_sssp_device(np_ctx *c, np_graph *g, int32_t src, const np_sched *sched, int32_t *d_dist, np_stats *stats) {
+    if (!d_dist) { set_error("null d_dist"); return NP_ERR_ARG; }
+    return solve_single(c, g, src, sched, d_dist, nullptr, false, stats);
+}
+
+int np_bfs(np_ctx *c, np_graph *g, int32_t src, const np_sched *sched, int32_t *level_out, np_stats *stats) {
+    if (!level_out) { set_error("null level_out"); return NP_ERR_ARG; }
+    return solve_single(c, g, src, sched, nullptr, level_out, true, stats);
+}
+
+int np_reached_work(np_ctx *c, np_graph *g, const int32_t *d_dist, int32_t inf, int64_t *n_r, int64_t *m_r) {
+    if (!c || !g || !d_dist) { set_error("null argument"); return NP_ERR_ARG; }
+    unsigned long long h[2] = {0, 0};
+    HIPCHK(launch_reached_work(g->row, d_dist, g->n, inf, c->d_work2, c->stream));
+    HIPCHK(hipMemcpyAsync(h, c->d_work2, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    int rc = ctx_sync(c);
+    if (rc) return rc;
+    if (n_r) *n_r = (int64_t)h[0];
+    if (m_r) *m_r = (int64_t)h[1];
+    return NP_OK;
+}
+
+int np_trace_get(np_ctx *c, np_trace_rec *recs, int32_t cap, int32_t *count) {
+    if (!c || !count) { set_error("null argument"); return NP_ERR_ARG; }
+    int k = std::min(c->last_iters, TRACE_CAP);
+    *count = k;
+    if (recs && cap > 0) {
+        int t = std::min(k, cap);
+        HIPCHK(hipMemcpyAsync(recs, c->trace, sizeof(np_trace_rec) * (size_t)t, hipMemcpyDeviceToHost, c->stream));
+        return ctx_sync(c);
+    }
+    return NP_OK;
+}
+
+// ---- batches -----------------------------------------------------------------------
+int np_batch_create(np_ctx *c, int32_t count, np_graph *const *graphs, const int32_t *srcs, int32_t unit,
+                    np_batch **out) {
+    if (!c || !graphs || !srcs || !out || count < 1) { set_error("bad batch arguments"); return NP_ERR_ARG; }
+    *out = nullptr;
+    HIPCHK(hipSetDevice(c->device));
+    std::vector<np_graph *> uniq;
+    std::vector<int> gi(count);
+    for (int i = 0; i < count; ++i) {
+        np_graph *g = graphs[i];
+        if (!g || g->ctx != c) { set_error("instance %d: bad graph", i); return NP_ERR_ARG; }
+        if (srcs[i] < 0 || srcs[i] >= g->n) { set_error("instance %d: source out of range", i); return NP_ERR_ARG; }
+        int k = -1;
+        for (size_t j = 0; j < uniq.size(); ++j) if (uniq[j] == g) k = (int)j;
+        if (k < 0) { k = (int)uniq.size(); uniq.push_back(g); }
+        gi[i] = k;
+    }
+    np_batch *b = new np_batch();
+    b->ctx = c; b->count = count; b->unit = unit ? 1 : 0;
+    Space &s = b->sp;
+    std::vector<long long> VB(uniq.size()), EB(uniq.size());
+    int rc = NP_OK;
+    if (uniq.size() == 1) {
+        b->row = uniq[0]->row; b->col = uniq[0]->col; b->w = uniq[0]->w;
+        b->owns_graph = false;
+        VB[0] = 0; EB[0] = 0;
+    } else {
+        long long nv = 0, ne = 0;
+        for (size_t j = 0; j < uniq.size(); ++j) { VB[j] = nv; EB[j] = ne; nv += uniq[j]->n + 1; ne += uniq[j]->m; }
+        if (ne > 0x7FFFFFFFll || nv > 0x7FFFFFFFll) { delete b; set_error("batch exceeds 2^31 edges"); return NP_ERR_CAPACITY; }
+        b->owns_graph = true;
+        hipError_t e = dalloc(&b->row, (size_t)nv);
+        if (e == hipSuccess) e = dalloc(&b->col, (size_t)ne);
+        if (e == hipSuccess) e = dalloc(&b->w, (size_t)ne);
+        for (size_t j = 0; j < uniq.size() && e == hipSuccess; ++j) {
+            np_graph *g = uniq[j];
+            e = launch_rebase_row(g->row, g->n, (int)EB[j], b->row + VB[j], c->stream);
+            if (e == hipSuccess && g->m)
+                e = hipMemcpyAsync(b->col + EB[j], g->col, sizeof(int) * (size_t)g->m, hipMemcpyDeviceToDevice, c->stream);
+            if (e == hipSuccess && g->m)
+                e = hipMemcpyAsync(b->w + EB[j], g->w, sizeof(int) * (size_t)g->m, hipMemcpyDeviceToDevice, c->stream);
+        }
+        if (e != hipSuccess) { rc = hip_fail(e, "batch graph concat", __LINE__); np_batch_free(b); return rc; }
+    }
+    // slots: instance i owns [base_i, base_i + pad(n_i)), pad to 2^LUT_SHIFT
+    const long long G = 1ll << LUT_SHIFT;
+    long long acc = 0;
+    b->base.resize(count); b->nverts.resize(count);
+    std::vector<int2> inst(count);
+    for (int i = 0; i < count; ++i) {
+        np_graph *g = uniq[gi[i]];
+        b->base[i] = (int)acc; b->nverts[i] = g->n;
+        inst[i] = make_int2((int)acc, (int)VB[gi[i]]);
+        for (int k = 0; k < NCLS; ++k) s.hist[k] += g->hist[k];
+        acc += ((long long)g->n + G - 1) / G * G;
+        if (acc > 0x7FFFFFFFll - G) { np_batch_free(b); set_error("batch exceeds 2^31 vertex slots"); return NP_ERR_CAPACITY; }
+    }
+    s.nslots = acc > 0 ? acc : G;
+    if ((rc = space_alloc(c, s, true))) { np_batch_free(b); return rc; }
+    hipError_t e = dalloc(&s.cls, (size_t)s.nslots);
+    s.own_cls = true;
+    if (e == hipSuccess) e = dalloc(&s.lut, (size_t)(s.nslots / G + 1));
+    if (e == hipSuccess) e = dalloc(&s.inst, (size_t)count);
+    if (e == hipSuccess) e = launch_fill_u32((unsigned *)s.cls, (s.nslots + 3) / 4, 0u, c->stream);
+    for (int i = 0; i < count && e == hipSuccess; ++i) {
+        np_graph *g = uniq[gi[i]];
+        e = launch_copy_cls(g->cls, s.cls + b->base[i], g->n, c->stream);
+        if (e == hipSuccess) e = launch_fill_lut(s.lut, b->base[i] / G, ((long long)g->n + G - 1) / G, i, c->stream);
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(s.inst, inst.data(), sizeof(int2) * (size_t)count, hipMemcpyHostToDevice, c->stream);
+    if (e != hipSuccess) { rc = hip_fail(e, "batch setup", __LINE__); np_batch_free(b); return rc; }
+    if ((rc = ensure_src(c, s, count))) { np_batch_free(b); return rc; }
+    std::vector<int> slots(count);
+    for (int i = 0; i < count; ++i) slots[i] = b->base[i] + srcs[i];
+    e = hipMemcpyAsync(s.src_slots, slots.data(), sizeof(int) * (size_t)count, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) { rc = hip_fail(e, "batch sources", __LINE__); np_batch_free(b); return rc; }
+    *out = b;
+    return NP_OK;
+}
+
+int np_batch_solve(np_ctx *c, np_batch *b, const np_sched *sched, np_stats *stats) {
+    if (!c || !b || b->ctx != c) { set_error("bad batch"); return NP_ERR_ARG; }
+    np_sched sch;
+    int rc = validate_sched(sched, &sch);
+    if (rc) return rc;
+    Space &s = b->sp;
+    const bool unit = b->unit != 0;
+    DevState st = make_state(c, s, b->row, b->col, unit ? nullptr : b->w, s.dist, unit ? UNDEFINED : INF);
+    int nmax = 1;
+    for (int i = 0; i < b->count; ++i) nmax = std::max(nmax, b->nverts[i]);
+    SchedDev sd = make_schedd(c, sch, nmax);
+    HIPCHK(launch_init_solve(st, s.src_slots, b->count, sd.push, c->stream));
+    return run_drive(c, s, st, sch, sd, true, unit, stats);
+}
+
+int np_batch_get(np_ctx *c, np_batch *b, int32_t i, int32_t *dist_out) {
+    if (!c || !b || !dist_out || i < 0 || i >= b->count) { set_error("bad argument"); return NP_ERR_ARG; }
+    HIPCHK(hipMemcpyAsync(dist_out, b->sp.dist + b->base[i], sizeof(int) * (size_t)b->nverts[i],
+                          hipMemcpyDeviceToHost, c->stream));
+    return ctx_sync(c);
+}
+
+int np_batch_device_dist(np_batch *b, int32_t i, const int32_t **d_dist) {
+    if (!b || !d_dist || i < 0 || i >= b->count) { set_error("bad argument"); return NP_ERR_ARG; }
+    *d_dist = b->sp.dist + b->base[i];
+    return NP_OK;
+}
+
+int np_batch_free(np_batch *b) {
+    if (!b) return NP_OK;
+    if (b->ctx) (void)hipSetDevice(b->ctx->device);
+    b->sp.release();
+    if (b->owns_graph) { dfree(b->row); dfree(b->col); dfree(b->w); }
+    delete b;
+    return NP_OK;
+}
+
+// ---- learned scheduler -------------------------------------------------------------
+int np_model_set(np_ctx *c, const float *params) {
+    if (!c || !params) { set_error("null argument"); return NP_ERR_ARG; }
+    for (int i = 0; i < NP_MLP_NPARAM; ++i)
+        if (!std::isfinite(params[i])) { set_error("parameter %d is not finite", i); return NP_ERR_ARG; }
+    HIPCHK(hipMemcpyAsync(c->model, params, sizeof(float) * NP_MLP_NPARAM, hipMemcpyHostToDevice, c->stream));
+    int rc = ctx_sync(c);
+    if (rc) return rc;
+    c->has_model = true;
+    c->adam_step = 0;
+    return NP_OK;
+}
+
+int np_model_get(np_ctx *c, float *params) {
+    if (!c || !params) { set_error("null argument"); return NP_ERR_ARG; }
+    if (!c->has_model) { set_error("no model set"); return NP_ERR_ARG; }
+    HIPCHK(hipMemcpyAsync(params, c->model, sizeof(float) * NP_MLP_NPARAM, hipMemcpyDeviceToHost, c->stream));
+    return ctx_sync(c);
+}
+
+struct TrainBufs {
+    float *x = nullptr, *t = nullptr, *work = nullptr, *grad = nullptr, *loss = nullptr;
+    int *lab = nullptr;
+    ~TrainBufs() { dfree(x); dfree(t); dfree(work); dfree(grad); dfree(loss); dfree(lab); }
+};
+
+static int upload_train(np_ctx *c, TrainBufs &tb, const float *x, const int32_t *label, const float *target,
+                        int batch, int nloss) {
+    HIPCHK(dalloc(&tb.x, (size_t)batch * NP_MLP_NIN));
+    HIPCHK(dalloc(&tb.t, (size_t)batch * 2));
+    HIPCHK(dalloc(&tb.lab, (size_t)batch));
+    HIPCHK(dalloc(&tb.work, mlp_work_floats(batch)));
+    HIPCHK(dalloc(&tb.grad, NP_MLP_NPARAM));
+    HIPCHK(dalloc(&tb.loss, (size_t)(nloss > 0 ? nloss : 1)));
+    HIPCHK(hipMemcpyAsync(tb.x, x, sizeof(float) * (size_t)batch * NP_MLP_NIN, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(tb.t, target, sizeof(float) * (size_t)batch * 2, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(tb.lab, label, sizeof(int) * (size_t)batch, hipMemcpyHostToDevice, c->stream));
+    return NP_OK;
+}
+
+static int check_train_args(np_ctx *c, const float *x, const int32_t *label, const float *target, int batch) {
+    if (!c || !x || !label || !target || batch < 1) { set_error("bad training arguments"); return NP_ERR_ARG; }
+    for (int i = 0; i < batch; ++i)
+        if (label[i] < 0 || label[i] > 2) { set_error("label %d out of range", i); return NP_ERR_ARG; }
+    return NP_OK;
+}
+
+static int ensure_model(np_ctx *c) {
+    if (c->has_model) return NP_OK;
+    std::vector<float> p(NP_MLP_NPARAM);
+    init_model_params(p.data());
+    return np_model_set(c, p.data());
+}
+
+int np_model_loss_grad(np_ctx *c, const float *x, const int32_t *label, const float *target, int32_t batch,
+                       float lam, float *loss, float *grad) {
+    int rc = check_train_args(c, x, label, target, batch);
+    if (rc) return rc;
+    if ((rc = ensure_model(c))) return rc;
+    TrainBufs tb;
+    if ((rc = upload_train(c, tb, x, label, target, batch, 1))) return rc;
+    HIPCHK(launch_mlp_loss_grad(c->model, tb.x, tb.lab, tb.t, batch, lam, tb.work, tb.grad, tb.loss, c->stream));
+    if (grad) HIPCHK(hipMemcpyAsync(grad, tb.grad, sizeof(float) * NP_MLP_NPARAM, hipMemcpyDeviceToHost, c->stream));
+    if (loss) HIPCHK(hipMemcpyAsync(loss, tb.loss, sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    return ctx_sync(c);
+}
+
+int np_model_train(np_ctx *c, const float *x, const int32_t *label, const float *target, int32_t batch,
+                   int32_t epochs, float lr, float lam, float *loss_out) {
+    int rc = check_train_args(c, x, label, target, batch);
+    if (rc) return rc;
+    if (epochs < 1) { set_error("epochs must be >= 1"); return NP_ERR_ARG; }
+    if ((rc = ensure_model(c))) return rc;
+    if (!c->adam_m) {
+        HIPCHK(dalloc(&c->adam_m, NP_MLP_NPARAM));
+        HIPCHK(dalloc(&c->adam_v, NP_MLP_NPARAM));
+    }
+    if (c->adam_step == 0) {
+        HIPCHK(launch_fill_i32((int *)c->adam_m, NP_MLP_NPARAM, 0, c->stream));
+        HIPCHK(launch_fill_i32((int *)c->adam_v, NP_MLP_NPARAM, 0, c->stream));
+    }
+    TrainBufs tb;
+    if ((rc = upload_train(c, tb, x, label, target, batch, epochs))) return rc;
+    for (int ep = 0; ep < epochs; ++ep) {
+        HIPCHK(launch_mlp_loss_grad(c->model, tb.x, tb.lab, tb.t, batch, lam, tb.work, tb.grad, tb.loss + ep, c->stream));
+        c->adam_step++;
+        HIPCHK(launch_adam(c->model, tb.grad, c->adam_m, c->adam_v, NP_MLP_NPARAM, lr, 0.9f, 0.999f, 1e-8f,
+                           c->adam_step, c->stream));
+    }
+    if (loss_out) HIPCHK(hipMemcpyAsync(loss_out, tb.loss, sizeof(float) * (size_t)epochs, hipMemcpyDeviceToHost, c->stream));
+    return ctx_sync(c);
+}
+
+int np_model_forward(np_ctx *c, const float *x, int32_t batch, float *out) {
+    if (!c || !x || !out || batch < 1) { set_error("bad arguments"); return NP_ERR_ARG; }
+    int rc = ensure_model(c);
+    if (rc) return rc;
+    float *dx = nullptr, *dout = nullptr;
+    HIPCHK(dalloc(&dx, (size_t)batch * NP_MLP_NIN));
+    hipError_t e = dalloc(&dout, (size_t)batch * 5);
+    if (e == hipSuccess) e = hipMemcpyAsync(dx, x, sizeof(float) * (size_t)batch * NP_MLP_NIN, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = launch_mlp_forward(c->model, dx, batch, dout, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, dout, sizeof(float) * (size_t)batch * 5, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    dfree(dx); dfree(dout);
+    if (e != hipSuccess) return hip_fail(e, "model forward", __LINE__);
+    return NP_OK;
+}
+
+// ---- stepping ----------------------------------------------------------------------
+static int step_state(np_ctx *c, DevState &st, SchedDev &sd) {
+    np_graph *g = c->step_g;
+    if (!g) { set_error("no stepped solve in progress"); return NP_ERR_ARG; }
+    st = make_state(c, g->sp, g->row, g->col, g->w, g->sp.dist, INF);
+    np_sched d;
+    np_sched_default(1, &d);
+    sd = make_schedd(c, d, g->n);
+    return NP_OK;
+}
+
+static int step_compact(np_ctx *c) {
+    DevState st; SchedDev sd;
+    int rc = step_state(c, st, sd);
+    if (rc) return rc;
+    HIPCHK(launch_compact(st, c->step_it, false, c->num_cus * 8, c->stream));
+    return ctx_sync(c);
+}
+
+int np_step_begin(np_ctx *c, np_graph *g, int32_t src, np_stats *stats) {
+    if (!c || !g || g->ctx != c) { set_error("bad arguments"); return NP_ERR_ARG; }
+    if (src < 0 || src >= g->n) { set_error("source out of range"); return NP_ERR_ARG; }
+    int rc = graph_space(c, g);
+    if (rc) return rc;
+    if (c->snap_n < g->n) {
+        dfree(c->snap_dist); dfree(c->snap_bm);
+        HIPCHK(dalloc(&c->snap_dist, (size_t)g->n));
+        HIPCHK(dalloc(&c->snap_bm, (size_t)(g->n + 31) / 32));
+        c->snap_n = g->n;
+    }
+    c->step_g = g;
+    c->step_it = 0;
+    DevState st; SchedDev sd;
+    if ((rc = step_state(c, st, sd))) return rc;
+    HIPCHK(hipMemcpyAsync(g->sp.src_slots, &src, sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(launch_init_solve(st, g->sp.src_slots, 1, 0, c->stream));
+    HIPCHK(launch_set_cfg(st, 1, 6, 6, c->stream));
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    return step_compact(c);
+}
+
+int np_step_frontier(np_ctx *c, int32_t *count, int32_t *done) {
+    if (!c || !count || !done) { set_error("null argument"); return NP_ERR_ARG; }
+    np_graph *g = c->step_g;
+    if (!g) { set_error("no stepped solve in progress"); return NP_ERR_ARG; }
+    HIPCHK(hipMemcpyAsync(count, g->sp.cnt + (c->step_it % 3) * NCLS, sizeof(int) * NCLS, hipMemcpyDeviceToHost, c->stream));
+    int rc = ctx_sync(c);
+    if (rc) return rc;
+    long long tot = 0;
+    for (int k = 1; k < NCLS; ++k) tot += count[k];
+    *done = tot == 0;
+    return NP_OK;
+}
+
+static int step_expand_timed(np_ctx *c, const np_sched *s, double *ms) {
+    DevState st; SchedDev sd;
+    int rc = step_state(c, st, sd);
+    if (rc) return rc;
+    np_sched sch;
+    if ((rc = validate_sched(s, &sch))) return rc;
+    int lv = sch.level >= 0 ? sch.level : 1;
+    HIPCHK(launch_set_cfg(st, lv, sch.log2_t_lo, sch.log2_t_hi, c->stream));
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    HIPCHK(launch_expand(st, sd, c->step_it, false, false, c->num_cus * 8, c->stream));
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    HIPCHK(hipEventSynchronize(c->ev1));
+    float f = 0.f;
+    HIPCHK(hipEventElapsedTime(&f, c->ev0, c->ev1));
+    if (ms) *ms = f;
+    return NP_OK;
+}
+
+int np_step_try(np_ctx *c, const np_sched *s, int32_t reps, double *ms) {
+    if (!c || !c->step_g) { set_error("no stepped solve in progress"); return NP_ERR_ARG; }
+    np_graph *g = c->step_g;
+    const size_t nw = (size_t)(g->n + 31) / 32;
+    HIPCHK(hipMemcpyAsync(c->snap_dist, g->sp.dist, sizeof(int) * (size_t)g->n, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->snap_bm, g->sp.bm[0], sizeof(unsigned) * nw, hipMemcpyDeviceToDevice, c->stream));
+    double best = 1e30;
+    for (int r = 0; r < (reps > 0 ? reps : 1); ++r) {
+        double t = 0;
+        int rc = step_expand_timed(c, s, &t);
+        if (rc) return rc;
+        best = std::min(best, t);
+        HIPCHK(hipMemcpyAsync(g->sp.dist, c->snap_dist, sizeof(int) * (size_t)g->n, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(g->sp.bm[0], c->snap_bm, sizeof(unsigned) * nw, hipMemcpyDeviceToDevice, c->stream));
+    }
+    if (ms) *ms = best;
+    return ctx_sync(c);
+}
+
+int np_step_advance(np_ctx *c, const np_sched *s, double *ms) {
+    if (!c || !c->step_g) { set_error("no stepped solve in progress"); return NP_ERR_ARG; }
+    int rc = step_expand_timed(c, s, ms);
+    if (rc) return rc;
+    c->step_it++;
+    return step_compact(c);
+}
+
+int np_step_end(np_ctx *c, int32_t *dist_out) {
+    if (!c || !c->step_g) { set_error("no stepped solve in progress"); return NP_ERR_ARG; }
+    np_graph *g = c->step_g;
+    if (dist_out) {
+        HIPCHK(hipMemcpyAsync(dist_out, g->sp.dist, sizeof(int) * (size_t)g->n, hipMemcpyDeviceToHost, c->stream));
+        int rc = ctx_sync(c);
+        if (rc) return rc;
+    }
+    c->step_g = nullptr;
+    return NP_OK;
+}
+
+// ---- autotuner ---------------------------------------------------------------------
+int np_autotune(np_ctx *c, np_graph *g, const int32_t *srcs, int32_t nsrc, const np_sched *base,
+                np_sched *best, double *best_ms) {
+    if (!c || !g || !srcs || nsrc < 1 || !best) { set_error("bad arguments"); return NP_ERR_ARG; }
+    np_sched b0;
+    int rc = validate_sched(base, &b0);
+    if (rc) return rc;
+    int occ = persistent_max_wgs_per_cu(false, false, b0.frontier == NP_FRONTIER_PUSH);
+    std::vector<int> wgs_opts = {0};
+    if (occ > 2) wgs_opts.push_back(2);
+    if (occ > 4) wgs_opts.push_back(4);
+    const int klos[] = {3, 4, 5, 6, 7};
+    const int khis[] = {6, 8, 10, 12, 14, 30};
+    double bestt = 1e30;
+    np_sched bs = b0;
+    std::vector<int> scratch((size_t)g->n);
+    int *d_dist = nullptr;
+    HIPCHK(dalloc(&d_dist, (size_t)g->n));
+    for (int lv = 0; lv <= 2; ++lv)
+        for (int klo : klos)
+            for (int khi : khis) {
+                if (lv == 0 && khi != khis[0]) continue;  // level 0 has no upper bin
+                if (khi < klo) continue;
+                for (int wg : wgs_opts) {
+                    np_sched s = b0;
+                    s.level = lv; s.log2_t_lo = klo; s.log2_t_hi = khi; s.wgs_per_cu = wg;
+                    s.trace = 0;
+                    double tot = 0;
+                    for (int i = 0; i < nsrc && rc == NP_OK; ++i) {
+                        np_stats st;
+                        rc = solve_single(c, g, srcs[i], &s, d_dist, nullptr, false, &st);
+                        tot += st.kernel_ms;
+                    }
+                    if (rc) { dfree(d_dist); return rc; }
+                    if (tot < bestt) { bestt = tot; bs = s; }
+                }
+            }
+    dfree(d_dist);
+    *best = bs;
+    if (best_ms) *best_ms = bestt / nsrc;
+    return NP_OK;
+}
+
+// ---- device generators -------------------------------------------------------------
+struct EdgeBufs {
+    uint32_t *s = nullptr, *d = nullptr;
+    int *w = nullptr;
+    ~EdgeBufs() { dfree(s); dfree(d); dfree(w); }
+};
+
+static int finish_generated(np_ctx *c, int n, long long m, EdgeBufs &eb, np_graph **out) {
+    GenOut go{};
+    hipError_t e = build_csr_device(eb.s, eb.d, eb.w, n, m, &go, c->stream);
+    if (e != hipSuccess) return hip_fail(e, "CSR build", __LINE__);
+    np_graph *g = new np_graph();
+    g->ctx = c; g->n = n; g->m = m; g->owns = true;
+    g->row = go.row; g->col = go.col; g->w = go.w;
+    int rc = finish_graph(c, g);
+    if (rc) { np_graph_free(g); return rc; }
+    *out = g;
+    return NP_OK;
+}
+
+int np_gen_rmat(np_ctx *c, int32_t scale, int32_t edge_factor, double a, double b, double cc, int32_t wmin,
+                int32_t wmax, uint64_t seed, int32_t permute, np_graph **out) {
+    if (!c || !out || scale < 1 || scale > 30 || edge_factor < 1 || wmax < wmin) {
+        set_error("bad R-MAT arguments"); return NP_ERR_ARG;
+    }
+    const int n = 1 << scale;
+    const long long m = (long long)edge_factor * n;
+    if (m > 0x7FFFFFFFll) { set_error("more than 2^31-1 edges"); return NP_ERR_CAPACITY; }
+    HIPCHK(hipSetDevice(c->device));
+    EdgeBufs eb;
+    HIPCHK(dalloc(&eb.s, (size_t)m)); HIPCHK(dalloc(&eb.d, (size_t)m)); HIPCHK(dalloc(&eb.w, (size_t)m));
+    HIPCHK(gen_rmat_edges(scale, m, seed, a, b, cc, permute, wmin, wmax, eb.s, eb.d, eb.w, c->stream));
+    return finish_generated(c, n, m, eb, out);
+}
+
+int np_gen_er(np_ctx *c, int32_t n, int64_t m, int32_t wmin, int32_t wmax, uint64_t seed, np_graph **out) {
+    if (!c || !out || n < 1 || m < 0 || m > 0x7FFFFFFFll || wmax < wmin) { set_error("bad ER arguments"); return NP_ERR_ARG; }
+    HIPCHK(hipSetDevice(c->device));
+    EdgeBufs eb;
+    HIPCHK(dalloc(&eb.s, (size_t)m)); HIPCHK(dalloc(&eb.d, (size_t)m)); HIPCHK(dalloc(&eb.w, (size_t)m));
+    HIPCHK(gen_er_edges(n, m, seed, wmin, wmax, eb.s, eb.d, eb.w, c->stream));
+    return finish_generated(c, n, m, eb, out);
+}
+
+int np_gen_grid(np_ctx *c, int32_t rows, int32_t cols, int64_t n_long, int32_t wmin, int32_t wmax,
+                uint64_t seed, np_graph **out) {
+    if (!c || !out || rows < 1 || cols < 1 || n_long < 0 || wmax < wmin) { set_error("bad grid arguments"); return NP_ERR_ARG; }
+    const long long n = (long long)rows * cols;
+    const long long grid_m = 2ll * ((long long)rows * (cols - 1) + (long long)cols * (rows - 1));
+    const long long m = grid_m + n_long;
+    if (n > 0x7FFFFFFFll || m > 0x7FFFFFFFll) { set_error("grid too large"); return NP_ERR_CAPACITY; }
+    HIPCHK(hipSetDevice(c->device));
+    EdgeBufs eb;
+    long long *off = nullptr;
+    HIPCHK(dalloc(&eb.s, (size_t)m)); HIPCHK(dalloc(&eb.d, (size_t)m)); HIPCHK(dalloc(&eb.w, (size_t)m));
+    HIPCHK(dalloc(&off, (size_t)n + 1));
+    hipError_t e = grid_offsets(rows, cols, off, c->stream);
+    if (e == hipSuccess) e = gen_grid_edges(rows, cols, n_long, seed, wmin, wmax, off, eb.s, eb.d, eb.w, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    dfree(off);
+    if (e != hipSuccess) return hip_fail(e, "grid generation", __LINE__);
+    return finish_generated(c, (int)n, m, eb, out);
+}
+
+}  // extern "C"

@@ -1,0 +1,238 @@
+// np_dropin.cpp -- the reference's C++ entry points (include/np_dropin.h) over the C ABI.
+//
+// Globals, readers and output helpers keep the reference's names, types and effects
+// (common/global.cpp); SSSP_GPU / BFS_REC_GPU keep their contract: read the global CSR
+// and `source`, write graph.costArray / graph.levelArray, accumulate the phase timers
+// (apps/sssp/src/gpu/sssp_wrapper.cu:548-583, apps/bfs-rec/src/gpu/bfs_rec_wrapper.cu:
+// 231-269).  Error behaviour follows the reference: a reader failure prints and exits
+// with status 0 (the EXIT macro, common/global.h:21-24); a device failure prints
+// "Error: ..." and exits with status 1 (cudaCheckError, sssp_wrapper.cu:50-56).
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <sys/time.h>
+
+#include "np_dropin.h"
+#include "np_sssp.h"
+
+#ifndef CONSOLIDATE_LEVEL
+#define CONSOLIDATE_LEVEL 1      // apps/sssp/src/gpu/Makefile:28
+#endif
+#ifndef BFS_CONSOLIDATE_LEVEL
+#define BFS_CONSOLIDATE_LEVEL 2  // apps/bfs-rec/src/gpu/bfs_rec_wrapper.cu:20-22
+#endif
+
+struct _GRAPH_ graph;
+int noNodeTotal = 0;
+int noEdgeTotal = 0;
+int source = 0;
+FILE *fp = NULL;
+int VERBOSE;
+int DEBUG;
+
+CONF config;
+double start_time = 0;
+double end_time = 0;
+double init_time = 0;
+double d_malloc_time = 0;
+double h2d_memcpy_time = 0;
+double ker_exe_time = 0;
+double d2h_memcpy_time = 0;
+
+namespace {
+
+int32_t *s_row = nullptr, *s_col = nullptr, *s_w = nullptr;  // CSR staged by readInput*
+np_ctx *s_ctx = nullptr;
+int s_ctx_dev = -1;
+
+void drop_staged() {
+    np_host_free(s_row); np_host_free(s_col); np_host_free(s_w);
+    s_row = s_col = s_w = nullptr;
+}
+
+int read_common(int fmt) {
+    drop_staged();
+    int32_t n = 0, src = 0;
+    int64_t m = 0;
+    if (!fp) {
+        fprintf(stderr, "info: %s:%d: %s", __FILE__, __LINE__, "NULL file pointer\n");
+        exit(0);
+    }
+    int rc = np_read_graph_stream(fmt, fp, &n, &m, &src, &s_row, &s_col, &s_w);
+    if (rc != NP_OK) {
+        fprintf(stderr, "info: %s:%d: %s\n", __FILE__, __LINE__, np_last_error());
+        exit(0);
+    }
+    noNodeTotal = n;
+    noEdgeTotal = (int)m;
+    if (fmt == 0) source = src;  // only DIMACS9 carries a source line
+    return 0;
+}
+
+void fail(const char *what) {
+    printf("Error: %s: %s\n", what, np_last_error());
+    exit(1);
+}
+
+np_ctx *ctx_for_device() {
+    if (s_ctx && s_ctx_dev == config.device_num) return s_ctx;
+    if (s_ctx) np_destroy(s_ctx);
+    s_ctx = nullptr;
+    if (np_create(config.device_num, &s_ctx) != NP_OK) fail("np_create");
+    s_ctx_dev = config.device_num;
+    return s_ctx;
+}
+
+int env_level(int dflt) {
+    const char *e = getenv("NP_CONSOLIDATE_LEVEL");
+    if (e && *e) {
+        int v = atoi(e);
+        if (v >= 0 && v <= 2) return v;
+    }
+    return dflt;
+}
+
+// Schedules for config.solution (apps/sssp/src/gpu/sssp.cpp:31-38).  Every solution
+// relaxes with the same arithmetic; they differ in how frontier vertices are mapped to
+// workers, which here is the bin configuration:
+//   0 thread queue, 1 dual queue, 2 shared delayed buffer, 3 global delayed buffer,
+//   4 per-vertex nested launch, 5 per-block nested launch, 6 consolidation.
+bool sssp_schedule(int solution, np_sched *s) {
+    switch (solution) {
+        case 0: np_sched_default(0, s); s->log2_t_lo = 30; s->log2_t_hi = 30; return true;
+        case 1: np_sched_default(1, s); return true;
+        case 2: np_sched_default(1, s); return true;
+        case 3: np_sched_default(2, s); return true;
+        case 4: np_sched_default(0, s); return true;
+        case 5: np_sched_default(1, s); return true;
+        case 6: np_sched_default(env_level(CONSOLIDATE_LEVEL), s); return true;
+        default: return false;
+    }
+}
+
+np_graph *upload_graph(np_ctx *c) {
+    np_graph *g = nullptr;
+    double t0 = gettime();
+    int rc = np_graph_upload(c, noNodeTotal, noEdgeTotal, graph.vertexArray, graph.edgeArray,
+                             graph.weightArray, &g);
+    double t1 = gettime();
+    if (rc != NP_OK) fail("graph upload");
+    h2d_memcpy_time += t1 - t0;
+    return g;
+}
+
+}  // namespace
+
+double gettime() {
+    struct timeval t;
+    gettimeofday(&t, NULL);
+    return t.tv_sec + t.tv_usec * 1e-6;
+}
+
+double gettime_ms() {
+    struct timeval t;
+    gettimeofday(&t, NULL);
+    return t.tv_sec * 1e+3 + t.tv_usec * 1e-3;
+}
+
+int readInputDIMACS9() { return read_common(0); }
+int readInputDIMACS10() { return read_common(1); }
+int readInputSLNDC() { return read_common(2); }
+int readInput() { return 0; }  // common/readInput.cpp:3-9
+
+int convertCSR() {
+    const int n = noNodeTotal, m = noEdgeTotal;
+    graph.vertexArray = new int[n + 1]();
+    graph.costArray = new int[n]();
+    graph.levelArray = new int[n]();
+    graph.edgeArray = new int[m]();
+    graph.weightArray = new int[m]();
+    graph.frontier = new char[n]();
+    graph.update = new char[n]();
+    graph.visited = new char[n]();
+    if (s_row) {
+        memcpy(graph.vertexArray, s_row, sizeof(int) * ((size_t)n + 1));
+        if (m) {
+            memcpy(graph.edgeArray, s_col, sizeof(int) * (size_t)m);
+            memcpy(graph.weightArray, s_w, sizeof(int) * (size_t)m);
+        }
+    }
+    graph.vertexArray[n] = noEdgeTotal;
+    if (VERBOSE) {
+        fprintf(stderr, "Node no: %d\n", noNodeTotal);
+        fprintf(stderr, "Edge no: %d\n", noEdgeTotal);
+    }
+    drop_staged();
+    return 0;
+}
+
+int outputCost(FILE *file) {
+    if (!file) file = stdout;
+    for (int i = 0; i < noNodeTotal; ++i)
+        fprintf(file, "From Node %d -> %d is %d\n", source + 1, i + 1, graph.costArray[i]);
+    return 0;
+}
+
+int clear() {
+    delete[] graph.vertexArray; graph.vertexArray = nullptr;
+    delete[] graph.edgeArray; graph.edgeArray = nullptr;
+    delete[] graph.weightArray; graph.weightArray = nullptr;
+    delete[] graph.frontier; graph.frontier = nullptr;
+    return 0;
+}
+
+void SSSP_GPU() {
+    start_time = gettime();
+    np_ctx *c = ctx_for_device();
+    end_time = gettime();
+    init_time += end_time - start_time;
+    if (DEBUG) fprintf(stderr, "Choose CUDA device: %d\n", config.device_num);
+    // host-side initialisation of prepare_gpu (sssp_wrapper.cu:92-96)
+    for (int i = 0; i < noNodeTotal; i++) graph.costArray[i] = NP_INF;
+    if (graph.update) graph.update[source] = 1;
+    graph.costArray[source] = 0;
+    np_graph *g = upload_graph(c);
+    np_sched s;
+    np_stats st;
+    memset(&st, 0, sizeof st);
+    start_time = gettime();
+    if (sssp_schedule(config.solution, &s)) {
+        if (np_sssp(c, g, source, &s, graph.costArray, &st) != NP_OK) fail("SSSP");
+        if (config.solution == 6) printf("iteration: %d\n", st.iterations);
+    }
+    end_time = gettime();
+    const double kernel_s = st.kernel_ms * 1e-3;
+    ker_exe_time += kernel_s;
+    d2h_memcpy_time += (end_time - start_time) - kernel_s > 0 ? (end_time - start_time) - kernel_s : 0;
+    if (DEBUG) fprintf(stderr, "Iteration: %d\n", st.iterations);
+    np_graph_free(g);
+}
+
+void BFS_REC_GPU() {
+    start_time = gettime_ms();
+    np_ctx *c = ctx_for_device();
+    end_time = gettime_ms();
+    init_time += end_time - start_time;
+    np_graph *g = nullptr;
+    start_time = gettime_ms();
+    if (np_graph_upload(c, noNodeTotal, noEdgeTotal, graph.vertexArray, graph.edgeArray,
+                        graph.weightArray, &g) != NP_OK)
+        fail("graph upload");
+    end_time = gettime_ms();
+    h2d_memcpy_time += end_time - start_time;
+    np_stats st;
+    memset(&st, 0, sizeof st);
+    double t0 = gettime_ms();
+    if (config.solution >= 0 && config.solution <= 3) {
+        // solutions 0 flat, 1 naive, 2 hierarchical, 3 consolidated
+        // (apps/bfs-rec/src/gpu/bfs_rec.cpp:31-34): one unit-weight solve each
+        np_sched s;
+        np_sched_default(config.solution == 3 ? env_level(BFS_CONSOLIDATE_LEVEL) : 1, &s);
+        if (np_bfs(c, g, source, &s, graph.levelArray, &st) != NP_OK) fail("BFS");
+        if (DEBUG) printf("===> GPU #%d - BFS levels in %d iterations.\n", config.solution + 1, st.iterations);
+    }
+    double t1 = gettime_ms();
+    ker_exe_time += st.kernel_ms;
+    d2h_memcpy_time += (t1 - t0) - st.kernel_ms > 0 ? (t1 - t0) - st.kernel_ms : 0;
+    np_graph_free(g);
+}

@@ -1,0 +1,113 @@
+// np_internal.h -- internal interfaces between the C-ABI layer (np_api.cpp) and the
+// HIP translation units (np_kernels.hip, np_gen.hip).  Not installed, not stable.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "np_common.h"
+#include "../../include/np_sssp.h"
+
+namespace np {
+
+// Device-side control block of one solve.
+struct Ctrl {
+    unsigned long long bar;   // grid barrier: cumulative arrivals
+    int error;                // NP_ERR_* raised by a kernel (0 = none)
+    int done;                 // frontier emptied
+    int iters;                // expansions performed
+    int it_base;              // graph drive: iteration index of the captured chunk's first slot
+    long long frontier_total; // sum of frontier sizes
+    int cfg_level, cfg_klo, cfg_khi, cfg_pad;  // schedule decided for the next expansion
+};
+
+// Everything an expansion / compaction touches.  Indices are "slots": the vertex ids of
+// a single solve, or instance-tagged vertex ids of a batch (slot = base_i + v).
+struct DevState {
+    const int *row;           // row offsets, indexed VB_i + v (graphs concatenated)
+    const int *col;           // local target ids
+    const int *w;             // weights (unused for unit-weight solves)
+    const unsigned char *cls; // out-degree class per slot
+    int *dist;                // per slot
+    unsigned *bm[2];          // update bitmaps per slot (scan uses bm[0]; push ping-pongs)
+    int *queue[2];            // frontier, segmented by class (seg_off)
+    const int *seg_off;       // [NCLS+1] class segment offsets inside a queue
+    int *cnt;                 // [3][NCLS] frontier class counters, rotating by iteration
+    const int *lut;           // batch: slot >> LUT_SHIFT -> instance
+    const int2 *inst;         // batch: per instance {slot base, row base}
+    long long nslots;
+    long long nwords;         // bitmap words
+    int inf;                  // unreached value written at init
+    Ctrl *ctrl;
+};
+
+struct SchedDev {
+    int learned;            // evaluate the MLP each iteration
+    int level, klo, khi;    // fixed schedule (the fallback when the MLP is unusable)
+    int push;               // NP_FRONTIER_PUSH
+    int max_iters;
+    int trace_cap;
+    int n_feat;             // vertex count used by the features
+    const float *mlp;       // NP_MLP_NPARAM device floats
+    np_trace_rec *trace;
+};
+
+// ---- kernels.hip: solve drivers ----------------------------------------------------
+// Launch shapes are chosen by the callers in np_api.cpp.
+hipError_t launch_fill_i32(int *p, long long n, int v, hipStream_t s);
+hipError_t launch_fill_u32(unsigned *p, long long n, unsigned v, hipStream_t s);
+hipError_t launch_degree_class(const int *row, int n, unsigned char *cls, int *hist, hipStream_t s);
+hipError_t launch_validate_csr(const int *row, const int *col, int n, long long m, int *bad,
+                               hipStream_t s);
+hipError_t launch_init_solve(const DevState &st, const int *d_src_slots, int nsrc, int push,
+                             hipStream_t s);
+hipError_t launch_reached_work(const int *row, const int *dist, int n, int inf,
+                               unsigned long long *out2, hipStream_t s);
+hipError_t launch_rebase_row(const int *row, int n, int add, int *out, hipStream_t s);
+hipError_t launch_fill_lut(int *lut, long long first, long long count, int inst, hipStream_t s);
+hipError_t launch_copy_cls(const unsigned char *src, unsigned char *dst, long long n,
+                           hipStream_t s);
+
+// Persistent drive: one cooperative launch.  `wgs` workgroups of 256 threads.
+hipError_t launch_persistent(const DevState &st, const SchedDev &sp, bool batch, bool unit,
+                             int wgs, hipStream_t s);
+int persistent_max_wgs_per_cu(bool batch, bool unit, bool push);
+
+// Host-loop / graph drive, one iteration = [compact] + decide + expand.
+hipError_t launch_compact(const DevState &st, int it_off, bool batch, int wgs, hipStream_t s);
+hipError_t launch_decide(const DevState &st, const SchedDev &sp, int it_off, hipStream_t s);
+hipError_t launch_expand(const DevState &st, const SchedDev &sp, int it_off, bool batch, bool unit,
+                         int wgs, hipStream_t s);
+hipError_t launch_advance(const DevState &st, int k, hipStream_t s);
+// Stepping: force the next expansion's schedule (writes ctrl->cfg_*).
+hipError_t launch_set_cfg(const DevState &st, int level, int klo, int khi, hipStream_t s);
+
+// ---- kernels.hip: scheduler MLP ------------------------------------------------------
+hipError_t launch_mlp_forward(const float *p, const float *x, int batch, float *out,
+                              hipStream_t s);
+// Per-sample forward+backward then per-parameter reduction: grad (NP_MLP_NPARAM) and the
+// mean loss (loss[0]).  work: scratch of mlp_work_floats(batch) floats.
+size_t mlp_work_floats(int batch);
+hipError_t launch_mlp_loss_grad(const float *p, const float *x, const int *label,
+                                const float *target, int batch, float lam, float *work,
+                                float *grad, float *loss, hipStream_t s);
+hipError_t launch_adam(float *p, const float *grad, float *m, float *v, int n, float lr,
+                       float b1, float b2, float eps, int step, hipStream_t s);
+
+// ---- gen.hip: generators and CSR construction ---------------------------------------
+struct GenOut { int *row; int *col; int *w; };
+// Builds CSR (reverse generation order per vertex) from an edge list in generation order.
+hipError_t build_csr_device(const uint32_t *src, const uint32_t *dst, const int *w, int n,
+                            long long m, GenOut *out, hipStream_t s);
+hipError_t gen_rmat_edges(int scale, long long m, uint64_t seed, double a, double b, double c,
+                          int permute, int wmin, int wmax, uint32_t *src, uint32_t *dst, int *w,
+                          hipStream_t s);
+hipError_t gen_er_edges(int n, long long m, uint64_t seed, int wmin, int wmax, uint32_t *src,
+                        uint32_t *dst, int *w, hipStream_t s);
+hipError_t gen_grid_edges(int rows, int cols, long long n_long, uint64_t seed, int wmin, int wmax,
+                          const long long *grid_off, uint32_t *src, uint32_t *dst, int *w,
+                          hipStream_t s);
+
+hipError_t grid_offsets(int rows, int cols, long long *grid_off, hipStream_t s);
+// Thresholds of rmat_edge from probabilities.
+void rmat_thresholds(double a, double b, double c, uint64_t *ta, uint64_t *tab, uint64_t *tabc);
+
+}  // namespace np

@@ -1,0 +1,962 @@
+// np_kernels.hip -- gfx950 kernels of the SSSP / recursive-BFS nested-parallelism path.
+//
+// The reference (apps/sssp/src/gpu/sssp_kernel.cu) expands each frontier vertex in a
+// thread and hands vertices of out-degree >= THRESHOLD to child grids launched from
+// the device, consolidated per warp (consolidate_warp_dp_kernel, :330-404), per block
+// (consolidate_block_dp_kernel, :407-481) or per grid (consolidate_grid_dp_kernel,
+// :484-541).  HIP has no device-side launch, so here the frontier is kept as a queue
+// segmented by out-degree class and every scope of consolidation is a mapping of those
+// segments onto workers of the same launch:
+//   lane bin       64 frontier vertices per wavefront, their edges spread evenly over the
+//                  64 lanes (prefix sum of degrees + owner search), coalesced CSR reads
+//   wavefront bin  one vertex per 64-lane wavefront (the reference's 64-thread child block)
+//   workgroup bin  one vertex per 256-thread workgroup (block-level consolidation)
+//   grid bin       a vertex split into 2048-edge chunks spread over every wavefront of
+//                  the grid (grid-level consolidation)
+// The relaxation is the reference's, bit for bit (sssp_kernel.cu:46-52):
+//   alt = dist[u] + w;  if (dist[v] > alt) atomicMin(&dist[v], alt)  -- int32.
+// A vertex whose distance improved is flagged in an update bitmap (the reference's
+// update[] byte array, one bit per vertex) and, for the next iteration, either compacted
+// from the bitmap (scan frontier, unorder_generateQueue_kernel :648-658) or appended by
+// the relaxing wavefront (push frontier).
+#include <hip/hip_runtime.h>
+#include "np_internal.h"
+
+namespace np {
+
+#define NP_AGENT __HIP_MEMORY_SCOPE_AGENT
+static constexpr int WG = 256;
+static constexpr int WAVES = WG / 64;
+static constexpr unsigned long long SPIN_LIMIT = 1ull << 24;
+
+// ------------------------------------------------------------------------------------
+// small helpers
+// ------------------------------------------------------------------------------------
+__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+
+// Data that one workgroup writes and another reads inside a persistent launch (queue
+// entries, bitmap words, counters, distances) only moves through device-coherent
+// accesses: relaxed agent-scope atomics, loads and stores (sc1 on gfx950: they bypass
+// the per-CU cache and the non-coherent L2 state of the other XCDs).  The grid barrier
+// then needs no L2 writeback or invalidate, which would also evict the read-only CSR.
+__device__ __forceinline__ int ld_coherent(const int *p) {
+    return __hip_atomic_load(const_cast<int *>(p), __ATOMIC_RELAXED, NP_AGENT);
+}
+__device__ __forceinline__ unsigned ld_coherent_u(const unsigned *p) {
+    return __hip_atomic_load(const_cast<unsigned *>(p), __ATOMIC_RELAXED, NP_AGENT);
+}
+__device__ __forceinline__ void st_coherent(int *p, int v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, NP_AGENT);
+}
+__device__ __forceinline__ void st_coherent_u(unsigned *p, unsigned v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, NP_AGENT);
+}
+
+__device__ __forceinline__ int wave_incl_scan(int x) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int t = __shfl_up(x, o);
+        if (lane >= o) x += t;
+    }
+    return x;
+}
+
+template <bool BATCH>
+__device__ __forceinline__ void decode(const DevState &st, int slot, int &B, int &VB) {
+    if (BATCH) {
+        int i = st.lut[slot >> LUT_SHIFT];
+        int2 d = st.inst[i];
+        B = d.x; VB = d.y;
+    } else {
+        B = 0; VB = 0;
+    }
+}
+
+// Grid-wide barrier of a cooperative launch.  Every wave first waits for its own
+// outstanding memory operations (s_waitcnt 0: its coherent stores and atomics have
+// completed), the workgroup meets at s_barrier, then one lane arrives on a cumulative
+// 64-bit counter (no reset) and spins with relaxed coherent loads.  The spin is
+// bounded: on expiry the block raises NP_ERR_TIMEOUT and every block leaves its loop at
+// the next barrier, so the grid always drains.
+__device__ bool grid_sync(Ctrl *ctrl, unsigned nblocks, unsigned long long &gen) {
+    __shared__ int s_ok;
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        gen += 1;
+        const unsigned long long target = gen * (unsigned long long)nblocks;
+        unsigned long long arrived =
+            __hip_atomic_fetch_add(&ctrl->bar, 1ull, __ATOMIC_RELAXED, NP_AGENT) + 1ull;
+        int ok = 1;
+        if (arrived < target) {
+            unsigned long long spins = 0;
+            while (__hip_atomic_load(&ctrl->bar, __ATOMIC_RELAXED, NP_AGENT) < target) {
+                __builtin_amdgcn_s_sleep(1);
+                if (++spins > SPIN_LIMIT) { ok = 0; break; }
+                if ((spins & 1023) == 0 && __hip_atomic_load(&ctrl->error, __ATOMIC_RELAXED, NP_AGENT)) {
+                    ok = 0; break;
+                }
+            }
+        }
+        if (!ok) {
+            int expected = 0;
+            __hip_atomic_compare_exchange_strong(&ctrl->error, &expected, (int)NP_ERR_TIMEOUT,
+                                                 __ATOMIC_RELAXED, __ATOMIC_RELAXED, NP_AGENT);
+        }
+        if (__hip_atomic_load(&ctrl->error, __ATOMIC_RELAXED, NP_AGENT)) ok = 0;
+        s_ok = ok;
+    }
+    __syncthreads();
+    return s_ok != 0;
+}
+
+// ------------------------------------------------------------------------------------
+// relaxation
+// ------------------------------------------------------------------------------------
+struct PushCtx {
+    unsigned *bm_next;  // bitmap receiving "improved" marks
+    unsigned *bm_cur;   // push: bitmap of the frontier being consumed (cleared here)
+    int *cnt_next;      // push: class counters of the next frontier
+    int *q_next;        // push: next frontier queue
+};
+
+// Tail of one relaxation after the cheap check passed: the atomicMin of the reference
+// and the update mark.  Returns true when the caller must append tslot (push only).
+template <bool PUSH>
+__device__ __forceinline__ bool relax_tail(const DevState &st, const PushCtx &pc, int tslot,
+                                           int alt, int &c_out) {
+    int old = atomicMin(&st.dist[tslot], alt);
+    if (alt >= old) return false;
+    int c = st.cls[tslot];
+    if (c == 0) return false;  // no out-edges: nothing to propagate
+    unsigned bit = 1u << (tslot & 31);
+    if (!PUSH) {
+        atomicOr(&pc.bm_next[tslot >> 5], bit);
+        return false;
+    }
+    unsigned prev = atomicOr(&pc.bm_next[tslot >> 5], bit);
+    c_out = c;
+    return (prev & bit) == 0;
+}
+
+// Wavefront-aggregated append into the class-segmented next frontier: one counter
+// atomic per (wavefront, class) group.
+__device__ __forceinline__ void push_wave(const DevState &st, const PushCtx &pc, bool want,
+                                          int slot, int c) {
+    unsigned long long mask = __ballot(want);
+    while (mask) {
+        int leader = __ffsll((long long)mask) - 1;
+        int cl = __shfl(c, leader);
+        unsigned long long same = __ballot(want && c == cl);
+        int base = 0;
+        if (lane_id() == leader) base = atomicAdd(&pc.cnt_next[cl], __popcll(same));
+        base = __shfl(base, leader);
+        if (want && c == cl) {
+            int pos = base + __popcll(same & __lanemask_lt());
+            st_coherent(pc.q_next + st.seg_off[cl] + pos, slot);
+            want = false;
+        }
+        mask &= ~same;
+    }
+}
+
+constexpr int UNR = 4;
+
+// Edges [e0, e1) of one vertex (slot base B, distance du), worker t of `stride`.
+// The loop trip count is uniform across the wavefront.
+template <bool UNIT, bool PUSH>
+__device__ __forceinline__ void relax_range(const DevState &st, const PushCtx &pc, int B, int e0,
+                                            int e1, int du, int t, int stride) {
+    for (int base = e0; base < e1; base += UNR * stride) {
+        int tv[UNR], alt[UNR], cur[UNR];
+        bool ok[UNR];
+#pragma unroll
+        for (int k = 0; k < UNR; ++k) {
+            int e = base + t + k * stride;
+            ok[k] = e < e1;
+            int ec = ok[k] ? e : e0;
+            tv[k] = B + st.col[ec];
+            alt[k] = du + (UNIT ? 1 : st.w[ec]);
+        }
+#pragma unroll
+        for (int k = 0; k < UNR; ++k) cur[k] = ok[k] ? ld_coherent(st.dist + tv[k]) : 0;
+#pragma unroll
+        for (int k = 0; k < UNR; ++k) {
+            bool want = false;
+            int c = 0;
+            if (ok[k] && alt[k] < cur[k]) want = relax_tail<PUSH>(st, pc, tv[k], alt[k], c);
+            if (PUSH) push_wave(st, pc, want, tv[k], c);
+        }
+    }
+}
+
+// Lane bin: 64 frontier vertices (queue[qidx .. qidx+nvalid)), edges balanced over lanes.
+template <bool BATCH, bool UNIT, bool PUSH>
+__device__ __forceinline__ void lane_tile(const DevState &st, const PushCtx &pc, const int *q,
+                                          long long qidx, int nvalid) {
+    const int lane = lane_id();
+    const bool valid = lane < nvalid;
+    int slot = valid ? ld_coherent(q + qidx + lane) : 0;
+    int B = 0, VB = 0, s = 0, d = 0, du = 0;
+    if (valid) {
+        decode<BATCH>(st, slot, B, VB);
+        int v = slot - B;
+        s = st.row[VB + v];
+        d = st.row[VB + v + 1] - s;
+        du = ld_coherent(st.dist + slot);
+        if (PUSH) st_coherent_u(pc.bm_cur + (slot >> 5), 0u);
+    }
+    const int incl = wave_incl_scan(d);
+    const int total = __shfl(incl, 63);
+    const int excl = incl - d;
+    if (total == 0) return;
+    // First edge of the tile (owner of index 0 = first lane with d > 0): a valid address
+    // for the lanes whose index runs past the end.
+    const int first_lane = __ffsll((long long)__ballot(d > 0)) - 1;
+    const int e_safe = __shfl(s, first_lane);
+    for (int base = 0; base < total; base += UNR * 64) {
+        int tv[UNR], alt[UNR], cur[UNR];
+        bool ok[UNR];
+#pragma unroll
+        for (int k = 0; k < UNR; ++k) {
+            const int kk = base + lane + k * 64;
+            ok[k] = kk < total;
+            // owner: the last lane whose exclusive offset is <= kk
+            int o = 0;
+#pragma unroll
+            for (int step = 32; step > 0; step >>= 1) {
+                int t = __shfl(excl, o + step);
+                if (t <= kk) o += step;
+            }
+            const int e = __shfl(s, o) + (kk - __shfl(excl, o));
+            const int ou = __shfl(du, o);
+            const int ob = BATCH ? __shfl(B, o) : 0;
+            const int ec = ok[k] ? e : e_safe;
+            tv[k] = ob + st.col[ec];
+            alt[k] = ou + (UNIT ? 1 : st.w[ec]);
+        }
+#pragma unroll
+        for (int k = 0; k < UNR; ++k) cur[k] = ok[k] ? ld_coherent(st.dist + tv[k]) : 0;
+#pragma unroll
+        for (int k = 0; k < UNR; ++k) {
+            bool want = false;
+            int c = 0;
+            if (ok[k] && alt[k] < cur[k]) want = relax_tail<PUSH>(st, pc, tv[k], alt[k], c);
+            if (PUSH) push_wave(st, pc, want, tv[k], c);
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// one expansion (all threads of every workgroup call it)
+// ------------------------------------------------------------------------------------
+// Per-workgroup item tables: exclusive prefix, by class, of the work items of each scope.
+struct ItemTables {
+    long long pre[4][NCLS + 1];  // 0 grid chunks, 1 workgroup items, 2 wavefront items, 3 lane tiles
+    int cnt[NCLS];
+    int seg[NCLS];
+};
+
+__device__ __forceinline__ int find_class(const long long *pre, long long idx) {
+    int c = 0;
+#pragma unroll
+    for (int step = 16; step > 0; step >>= 1)
+        if (c + step <= NCLS - 1 && pre[c + step] <= idx) c += step;
+    return c;
+}
+
+// Thread 0 builds the tables from the frontier's class counts (already in tb.cnt).
+__device__ __noinline__ void build_tables(ItemTables &tb, const Cfg &cfg) {
+    long long acc[4] = {0, 0, 0, 0};
+    for (int c = 0; c < NCLS; ++c) {
+        for (int t = 0; t < 4; ++t) tb.pre[t][c] = acc[t];
+        long long n = c == 0 ? 0 : tb.cnt[c];
+        if (c >= cfg.b3) acc[0] += n * grid_chunks(c);
+        else if (c >= cfg.b2) acc[1] += n;
+        else if (c >= cfg.b1) acc[2] += n;
+        else acc[3] += (n + 63) / 64;
+    }
+    for (int t = 0; t < 4; ++t) tb.pre[t][NCLS] = acc[t];
+}
+
+template <bool BATCH, bool UNIT, bool PUSH>
+__device__ void expansion(const DevState &st, const PushCtx &pc, const int *q, ItemTables &tb) {
+    const long long G = tb.pre[0][NCLS], K = tb.pre[1][NCLS], W = tb.pre[2][NCLS],
+                    T = tb.pre[3][NCLS];
+    // workgroup items: the whole workgroup strides one vertex's edges
+    for (long long k = blockIdx.x; k < K; k += gridDim.x) {
+        int c = find_class(tb.pre[1], k);
+        long long qi = tb.seg[c] + (k - tb.pre[1][c]);
+        int slot = ld_coherent(q + qi);
+        int B, VB;
+        decode<BATCH>(st, slot, B, VB);
+        int v = slot - B;
+        int e0 = st.row[VB + v], e1 = st.row[VB + v + 1];
+        int du = ld_coherent(st.dist + slot);
+        if (PUSH && threadIdx.x == 0) st_coherent_u(pc.bm_cur + (slot >> 5), 0u);
+        relax_range<UNIT, PUSH>(st, pc, B, e0, e1, du, threadIdx.x, WG);
+    }
+    // wavefront-granular items: grid chunks, wavefront items, lane tiles, round-robin
+    const long long NW = (long long)gridDim.x * WAVES;
+    const long long gw = (long long)blockIdx.x * WAVES + (threadIdx.x >> 6);
+    const long long U = G + W + T;
+    for (long long u = gw; u < U; u += NW) {
+        if (u < G) {
+            int c = find_class(tb.pre[0], u);
+            long long r = u - tb.pre[0][c];
+            int sh = c > GRID_CHUNK_SHIFT ? c - GRID_CHUNK_SHIFT : 0;
+            long long qi = tb.seg[c] + (r >> sh);
+            int j = (int)(r & ((1ll << sh) - 1));
+            int slot = ld_coherent(q + qi);
+            int B, VB;
+            decode<BATCH>(st, slot, B, VB);
+            int v = slot - B;
+            int s = st.row[VB + v], e = st.row[VB + v + 1];
+            int e0 = s + (j << GRID_CHUNK_SHIFT);
+            int e1 = e0 + (1 << GRID_CHUNK_SHIFT);
+            if (e1 > e) e1 = e;
+            if (PUSH && j == 0 && lane_id() == 0) st_coherent_u(pc.bm_cur + (slot >> 5), 0u);
+            if (e0 < e1) {
+                int du = ld_coherent(st.dist + slot);
+                relax_range<UNIT, PUSH>(st, pc, B, e0, e1, du, lane_id(), 64);
+            }
+        } else if (u < G + W) {
+            long long i = u - G;
+            int c = find_class(tb.pre[2], i);
+            long long qi = tb.seg[c] + (i - tb.pre[2][c]);
+            int slot = ld_coherent(q + qi);
+            int B, VB;
+            decode<BATCH>(st, slot, B, VB);
+            int v = slot - B;
+            int e0 = st.row[VB + v], e1 = st.row[VB + v + 1];
+            int du = ld_coherent(st.dist + slot);
+            if (PUSH && lane_id() == 0) st_coherent_u(pc.bm_cur + (slot >> 5), 0u);
+            relax_range<UNIT, PUSH>(st, pc, B, e0, e1, du, lane_id(), 64);
+        } else {
+            long long t = u - G - W;
+            int c = find_class(tb.pre[3], t);
+            long long ti = t - tb.pre[3][c];
+            long long first = ti * 64;
+            int nvalid = (int)((long long)tb.cnt[c] - first < 64 ? (long long)tb.cnt[c] - first : 64);
+            lane_tile<BATCH, UNIT, PUSH>(st, pc, q, tb.seg[c] + first, nvalid);
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// compaction (scan frontier): update bitmap -> class-segmented queue
+// ------------------------------------------------------------------------------------
+__device__ __forceinline__ void compaction_impl(unsigned *bm, const unsigned char *cls, const int *seg_off,
+                                                long long nwords, int *cnt_out, int *q_out) {
+    __shared__ int s_lc[NCLS], s_lb[NCLS];
+    for (long long wb = (long long)blockIdx.x * WG; wb < nwords; wb += (long long)gridDim.x * WG) {
+        if (threadIdx.x < NCLS) s_lc[threadIdx.x] = 0;
+        __syncthreads();
+        const long long wi = wb + threadIdx.x;
+        const unsigned word0 = wi < nwords ? ld_coherent_u(bm + wi) : 0u;
+        unsigned word = word0;
+        while (word) {
+            int b = __ffs(word) - 1;
+            word &= word - 1;
+            int c = cls[wi * 32 + b];
+            if (c) atomicAdd(&s_lc[c], 1);
+        }
+        __syncthreads();
+        if (threadIdx.x < NCLS) {
+            int k = s_lc[threadIdx.x];
+            s_lb[threadIdx.x] = k ? atomicAdd(&cnt_out[threadIdx.x], k) : 0;
+            s_lc[threadIdx.x] = 0;
+        }
+        __syncthreads();
+        word = word0;
+        while (word) {
+            int b = __ffs(word) - 1;
+            word &= word - 1;
+            long long slot = wi * 32 + b;
+            int c = cls[slot];
+            if (c) {
+                int p = atomicAdd(&s_lc[c], 1);
+                st_coherent(q_out + seg_off[c] + s_lb[c] + p, (int)slot);
+            }
+        }
+        if (word0) st_coherent_u(bm + wi, 0u);
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// schedule decision (all threads of a workgroup; the MLP is evaluated redundantly per
+// workgroup so every workgroup reaches the same decision without another barrier)
+// ------------------------------------------------------------------------------------
+__device__ __noinline__ Cfg decide(const SchedDev &sp, const int *cnt, int it) {
+    __shared__ float s_x[NP_MLP_NIN], s_h1[32], s_h2[32], s_o[5];
+    Cfg fixed = make_cfg(sp.level, sp.klo, sp.khi);
+    if (!sp.learned || !sp.mlp) return fixed;
+    if (threadIdx.x == 0) features(cnt, it, sp.n_feat, s_x);
+    __syncthreads();
+    const float *W1 = sp.mlp, *b1 = W1 + 32 * NP_MLP_NIN, *W2 = b1 + 32, *b2 = W2 + 32 * 32;
+    const float *W3 = b2 + 32, *b3 = W3 + 5 * 32;
+    const int j = threadIdx.x;
+    if (j < 32) {
+        float a = b1[j];
+#pragma unroll 2
+        for (int i = 0; i < NP_MLP_NIN; ++i) a += W1[j * NP_MLP_NIN + i] * s_x[i];
+        s_h1[j] = a > 0.f ? a : 0.f;
+    }
+    __syncthreads();
+    if (j < 32) {
+        float a = b2[j];
+#pragma unroll 2
+        for (int i = 0; i < 32; ++i) a += W2[j * 32 + i] * s_h1[i];
+        s_h2[j] = a > 0.f ? a : 0.f;
+    }
+    __syncthreads();
+    if (j < 5) {
+        float a = b3[j];
+#pragma unroll 2
+        for (int i = 0; i < 32; ++i) a += W3[j * 32 + i] * s_h2[i];
+        s_o[j] = a;
+    }
+    __syncthreads();
+    float o0 = s_o[0], o1 = s_o[1], o2 = s_o[2], o3 = s_o[3], o4 = s_o[4];
+    __syncthreads();
+    if (!(isfinite(o0) && isfinite(o1) && isfinite(o2) && isfinite(o3) && isfinite(o4)) ||
+        fabsf(o3) > 1e6f || fabsf(o4) > 1e6f)
+        return fixed;  // fallback: the fixed-threshold heuristic
+    int level = (o1 > o0) ? ((o2 > o1) ? 2 : 1) : ((o2 > o0) ? 2 : 0);
+    return make_cfg(level, (int)rintf(o3), (int)rintf(o4));
+}
+
+__device__ __noinline__ void record_trace(const SchedDev &sp, int it, const Cfg &cfg, const int *cnt,
+                             long long t0) {
+    if (!sp.trace || it >= sp.trace_cap) return;
+    np_trace_rec *r = sp.trace + it;
+    r->iteration = it;
+    r->level = cfg.level;
+    r->log2_t_lo = cfg.klo;
+    r->log2_t_hi = cfg.khi;
+    for (int c = 0; c < NCLS; ++c) r->count[c] = cnt[c];
+    r->t_begin = t0;
+    r->t_end = t0;
+}
+
+// ------------------------------------------------------------------------------------
+// persistent drive
+// ------------------------------------------------------------------------------------
+template <bool BATCH, bool UNIT, bool PUSH>
+__global__ __launch_bounds__(WG) void k_persistent(DevState st, SchedDev sp) {
+    __shared__ ItemTables tb;
+    __shared__ int s_total;
+    Ctrl *ctrl = st.ctrl;
+    unsigned long long gen = 0;
+    const unsigned nb = gridDim.x;
+    long long t_prev = 0;
+    for (int it = 0;; ++it) {
+        int *cnt_cur = st.cnt + (it % 3) * NCLS;
+        int *cnt_zero = st.cnt + ((it + 2) % 3) * NCLS;
+        int *q_cur = st.queue[it & 1];
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            long long now = (long long)wall_clock64();
+            if (it > 0 && sp.trace && it - 1 < sp.trace_cap) sp.trace[it - 1].t_end = now;
+            t_prev = now;
+        }
+        if (blockIdx.x == 0 && threadIdx.x < NCLS) st_coherent(cnt_zero + threadIdx.x, 0);
+        if (!PUSH) {
+            compaction_impl(st.bm[0], st.cls, st.seg_off, st.nwords, cnt_cur, q_cur);
+            if (!grid_sync(ctrl, nb, gen)) return;
+        }
+        if (threadIdx.x < NCLS) {
+            tb.cnt[threadIdx.x] = ld_coherent(cnt_cur + threadIdx.x);
+            tb.seg[threadIdx.x] = st.seg_off[threadIdx.x];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            long long tot = 0;
+            for (int c = 1; c < NCLS; ++c) tot += tb.cnt[c];
+            s_total = tot > 0x7FFFFFFF ? 0x7FFFFFFF : (int)tot;
+        }
+        __syncthreads();
+        const int total = s_total;
+        if (total == 0 || (sp.max_iters > 0 && it >= sp.max_iters)) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                ctrl->iters = it;
+                ctrl->done = total == 0 ? 1 : 0;
+                if (total != 0) ctrl->error = NP_ERR_NOT_CONVERGED;
+            }
+            return;
+        }
+        Cfg cfg = decide(sp, tb.cnt, it);
+        if (threadIdx.x == 0) build_tables(tb, cfg);
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            ctrl->frontier_total += total;
+            record_trace(sp, it, cfg, tb.cnt, t_prev);
+        }
+        __syncthreads();
+        PushCtx pc;
+        pc.bm_next = PUSH ? st.bm[(it + 1) & 1] : st.bm[0];
+        pc.bm_cur = st.bm[it & 1];
+        pc.cnt_next = st.cnt + ((it + 1) % 3) * NCLS;
+        pc.q_next = st.queue[(it + 1) & 1];
+        expansion<BATCH, UNIT, PUSH>(st, pc, q_cur, tb);
+        if (!grid_sync(ctrl, nb, gen)) return;
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// host-loop / graph drive kernels
+// ------------------------------------------------------------------------------------
+__global__ __launch_bounds__(WG) void k_compact(DevState st, int it_off) {
+    Ctrl *ctrl = st.ctrl;
+    if (ld_coherent(&ctrl->done) || ld_coherent(&ctrl->error)) return;
+    const int it = ld_coherent(&ctrl->it_base) + it_off;
+    if (blockIdx.x == 0 && threadIdx.x < NCLS) st.cnt[((it + 2) % 3) * NCLS + threadIdx.x] = 0;
+    compaction_impl(st.bm[0], st.cls, st.seg_off, st.nwords, st.cnt + (it % 3) * NCLS, st.queue[it & 1]);
+}
+
+__global__ __launch_bounds__(WG) void k_decide(DevState st, SchedDev sp, int it_off) {
+    __shared__ int s_cnt[NCLS];
+    Ctrl *ctrl = st.ctrl;
+    if (ld_coherent(&ctrl->done) || ld_coherent(&ctrl->error)) return;
+    const int it = ld_coherent(&ctrl->it_base) + it_off;
+    if (threadIdx.x < NCLS) s_cnt[threadIdx.x] = ld_coherent(st.cnt + (it % 3) * NCLS + threadIdx.x);
+    __syncthreads();
+    long long tot = 0;
+    for (int c = 1; c < NCLS; ++c) tot += s_cnt[c];
+    if (tot == 0 || (sp.max_iters > 0 && it >= sp.max_iters)) {
+        if (threadIdx.x == 0) {
+            ctrl->iters = it;
+            if (tot == 0) ctrl->done = 1;
+            else ctrl->error = NP_ERR_NOT_CONVERGED;
+        }
+        return;
+    }
+    Cfg cfg = decide(sp, s_cnt, it);
+    if (threadIdx.x == 0) {
+        if (ctrl->cfg_pad == 0) {  // cfg_pad != 0: a forced schedule (stepping) is kept
+            ctrl->cfg_level = cfg.level;
+            ctrl->cfg_klo = cfg.klo;
+            ctrl->cfg_khi = cfg.khi;
+        } else {
+            cfg = make_cfg(ctrl->cfg_level, ctrl->cfg_klo, ctrl->cfg_khi);
+        }
+        ctrl->frontier_total += tot;
+        ctrl->iters = it + 1;
+        record_trace(sp, it, cfg, s_cnt, (long long)wall_clock64());
+    }
+}
+
+template <bool BATCH, bool UNIT, bool PUSH>
+__global__ __launch_bounds__(WG) void k_expand(DevState st, int it_off) {
+    __shared__ ItemTables tb;
+    Ctrl *ctrl = st.ctrl;
+    if (ld_coherent(&ctrl->done) || ld_coherent(&ctrl->error)) return;
+    const int it = ld_coherent(&ctrl->it_base) + it_off;
+    if (threadIdx.x < NCLS) {
+        tb.cnt[threadIdx.x] = ld_coherent(st.cnt + (it % 3) * NCLS + threadIdx.x);
+        tb.seg[threadIdx.x] = st.seg_off[threadIdx.x];
+    }
+    if (PUSH && blockIdx.x == 0 && threadIdx.x < NCLS) st.cnt[((it + 2) % 3) * NCLS + threadIdx.x] = 0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        Cfg cfg = make_cfg(ld_coherent(&ctrl->cfg_level), ld_coherent(&ctrl->cfg_klo),
+                           ld_coherent(&ctrl->cfg_khi));
+        build_tables(tb, cfg);
+    }
+    __syncthreads();
+    PushCtx pc;
+    pc.bm_next = PUSH ? st.bm[(it + 1) & 1] : st.bm[0];
+    pc.bm_cur = st.bm[it & 1];
+    pc.cnt_next = st.cnt + ((it + 1) % 3) * NCLS;
+    pc.q_next = st.queue[(it + 1) & 1];
+    expansion<BATCH, UNIT, PUSH>(st, pc, st.queue[it & 1], tb);
+}
+
+__global__ void k_advance(DevState st, int k) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) st.ctrl->it_base += k;
+}
+
+__global__ void k_set_cfg(DevState st, int level, int klo, int khi) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        st.ctrl->cfg_level = level;
+        st.ctrl->cfg_klo = klo;
+        st.ctrl->cfg_khi = khi;
+        st.ctrl->cfg_pad = 1;
+    }
+}
+
+// ------------------------------------------------------------------------------------
+// setup kernels
+// ------------------------------------------------------------------------------------
+__global__ void k_fill_i32(int *p, long long n, int v) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+__global__ void k_fill_u32(unsigned *p, long long n, unsigned v) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ __launch_bounds__(WG) void k_degree_class(const int *row, int n, unsigned char *cls,
+                                                     int *hist) {
+    __shared__ int h[NCLS];
+    if (threadIdx.x < NCLS) h[threadIdx.x] = 0;
+    __syncthreads();
+    for (long long v = (long long)blockIdx.x * WG + threadIdx.x; v < n; v += (long long)gridDim.x * WG) {
+        int c = deg_class(row[v + 1] - row[v]);
+        cls[v] = (unsigned char)c;
+        atomicAdd(&h[c], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x < NCLS && h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
+}
+
+// CSR well-formedness, so that no kernel can index outside the arrays:
+// row[0] == 0, row non-decreasing, row[n] == m, 0 <= col < n.  bad[0] counts violations.
+__global__ void k_validate_csr(const int *row, const int *col, int n, long long m, int *bad) {
+    int local = 0;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i <= n; i += stride) {
+        int r = row[i];
+        if (i == 0 && r != 0) local++;
+        if (i == n && (long long)r != m) local++;
+        if (i < n && row[i + 1] < r) local++;
+    }
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < m; e += stride) {
+        int c = col[e];
+        if (c < 0 || c >= n) local++;
+    }
+    if (local) atomicAdd(bad, local);
+}
+
+template <bool PUSH>
+__global__ void k_init_sources(DevState st, const int *srcs, int nsrc) {
+    if (blockIdx.x != 0) return;
+    for (int i = threadIdx.x; i < nsrc; i += blockDim.x) {
+        int slot = srcs[i];
+        st.dist[slot] = 0;
+        int c = st.cls[slot];
+        if (c == 0) continue;
+        atomicOr(&st.bm[0][slot >> 5], 1u << (slot & 31));
+        if (PUSH) {
+            int p = atomicAdd(&st.cnt[c], 1);  // counters of iteration 0
+            st.queue[0][st.seg_off[c] + p] = slot;
+        }
+    }
+}
+
+__global__ __launch_bounds__(WG) void k_reached_work(const int *row, const int *dist, int n, int inf,
+                                                     unsigned long long *out2) {
+    __shared__ unsigned long long sa[WG], sb[WG];
+    unsigned long long a = 0, b = 0;
+    for (long long v = (long long)blockIdx.x * WG + threadIdx.x; v < n; v += (long long)gridDim.x * WG) {
+        if (dist[v] != inf) { a++; b += (unsigned long long)(row[v + 1] - row[v]); }
+    }
+    sa[threadIdx.x] = a; sb[threadIdx.x] = b;
+    __syncthreads();
+    for (int o = WG / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) { sa[threadIdx.x] += sa[threadIdx.x + o]; sb[threadIdx.x] += sb[threadIdx.x + o]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { atomicAdd(&out2[0], sa[0]); atomicAdd(&out2[1], sb[0]); }
+}
+
+__global__ void k_rebase_row(const int *row, int n, int add, int *out) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i <= n;
+         i += (long long)gridDim.x * blockDim.x)
+        out[i] = row[i] + add;
+}
+__global__ void k_fill_lut(int *lut, long long first, long long count, int inst) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (long long)gridDim.x * blockDim.x)
+        lut[first + i] = inst;
+}
+__global__ void k_copy_u8(const unsigned char *src, unsigned char *dst, long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+// ------------------------------------------------------------------------------------
+// scheduler MLP (fp32): forward, per-sample backward, per-parameter reduction, Adam
+// ------------------------------------------------------------------------------------
+constexpr int H = 32, NO = 5, NI = NP_MLP_NIN;
+// per-sample scratch: x copy not needed; h1, h2 (post-ReLU), g1, g2 (pre-activation grads),
+// go (output grads), loss
+constexpr int PER_SAMPLE = H + H + H + H + NO + 1;
+
+__device__ __forceinline__ void mlp_fwd_sample(const float *p, const float *x, float *a1, float *h1,
+                                               float *a2, float *h2, float *o) {
+    const float *W1 = p, *b1 = W1 + H * NI, *W2 = b1 + H, *b2 = W2 + H * H, *W3 = b2 + H,
+                *b3 = W3 + NO * H;
+    for (int j = 0; j < H; ++j) {
+        float a = b1[j];
+        for (int i = 0; i < NI; ++i) a += W1[j * NI + i] * x[i];
+        a1[j] = a; h1[j] = a > 0.f ? a : 0.f;
+    }
+    for (int j = 0; j < H; ++j) {
+        float a = b2[j];
+        for (int i = 0; i < H; ++i) a += W2[j * H + i] * h1[i];
+        a2[j] = a; h2[j] = a > 0.f ? a : 0.f;
+    }
+    for (int j = 0; j < NO; ++j) {
+        float a = b3[j];
+        for (int i = 0; i < H; ++i) a += W3[j * H + i] * h2[i];
+        o[j] = a;
+    }
+}
+
+__global__ void k_mlp_forward(const float *p, const float *x, int batch, float *out) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= batch) return;
+    float a1[H], h1[H], a2[H], h2[H], o[NO];
+    mlp_fwd_sample(p, x + (long long)s * NI, a1, h1, a2, h2, o);
+    for (int j = 0; j < NO; ++j) out[(long long)s * NO + j] = o[j];
+}
+
+// One thread per sample: forward, softmax-CE + lam * squared error, and the gradients of
+// the pre-activations (stored for the reduction).
+__global__ void k_mlp_sample_grad(const float *p, const float *x, const int *label,
+                                  const float *target, int batch, float lam, float *work) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= batch) return;
+    const float *W2 = p + H * NI + H, *W3 = W2 + H * H + H;
+    float a1[H], h1[H], a2[H], h2[H], o[NO], go[NO];
+    mlp_fwd_sample(p, x + (long long)s * NI, a1, h1, a2, h2, o);
+    const float inv = 1.0f / (float)batch;
+    float mx = fmaxf(o[0], fmaxf(o[1], o[2]));
+    float e0 = expf(o[0] - mx), e1 = expf(o[1] - mx), e2 = expf(o[2] - mx);
+    float z = e0 + e1 + e2;
+    float pr[3] = {e0 / z, e1 / z, e2 / z};
+    int lb = label[s];
+    float loss = -logf(fmaxf(pr[lb], 1e-30f));
+    for (int j = 0; j < 3; ++j) go[j] = (pr[j] - (j == lb ? 1.f : 0.f)) * inv;
+    for (int j = 0; j < 2; ++j) {
+        float d = o[3 + j] - target[(long long)s * 2 + j];
+        loss += lam * d * d;
+        go[3 + j] = 2.f * lam * d * inv;
+    }
+    float *wk = work + (long long)s * PER_SAMPLE;
+    float *wh1 = wk, *wh2 = wk + H, *wg1 = wk + 2 * H, *wg2 = wk + 3 * H, *wgo = wk + 4 * H;
+    float g2[H];
+    for (int i = 0; i < H; ++i) {
+        float g = 0.f;
+        for (int j = 0; j < NO; ++j) g += W3[j * H + i] * go[j];
+        g2[i] = a2[i] > 0.f ? g : 0.f;
+    }
+    for (int i = 0; i < H; ++i) {
+        float g = 0.f;
+        for (int j = 0; j < H; ++j) g += W2[j * H + i] * g2[j];
+        wg1[i] = a1[i] > 0.f ? g : 0.f;
+        wh1[i] = h1[i];
+        wh2[i] = h2[i];
+        wg2[i] = g2[i];
+    }
+    for (int j = 0; j < NO; ++j) wgo[j] = go[j];
+    wk[4 * H + NO] = loss * inv;
+}
+
+// One thread per parameter: sum of the per-sample contributions (deterministic order).
+__global__ void k_mlp_reduce(const float *x, int batch, const float *work, float *grad,
+                             float *loss) {
+    int pidx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pidx > NP_MLP_NPARAM) return;
+    if (pidx == NP_MLP_NPARAM) {
+        float l = 0.f;
+        for (int s = 0; s < batch; ++s) l += work[(long long)s * PER_SAMPLE + 4 * H + NO];
+        loss[0] = l;
+        return;
+    }
+    // locate the parameter
+    int q = pidx;
+    float acc = 0.f;
+    if (q < H * NI) {  // W1[j][i] += g1[j] * x[i]
+        int j = q / NI, i = q % NI;
+        for (int s = 0; s < batch; ++s) acc += work[(long long)s * PER_SAMPLE + 2 * H + j] * x[(long long)s * NI + i];
+    } else if ((q -= H * NI) < H) {  // b1
+        for (int s = 0; s < batch; ++s) acc += work[(long long)s * PER_SAMPLE + 2 * H + q];
+    } else if ((q -= H) < H * H) {  // W2[j][i] += g2[j] * h1[i]
+        int j = q / H, i = q % H;
+        for (int s = 0; s < batch; ++s) acc += work[(long long)s * PER_SAMPLE + 3 * H + j] * work[(long long)s * PER_SAMPLE + i];
+    } else if ((q -= H * H) < H) {  // b2
+        for (int s = 0; s < batch; ++s) acc += work[(long long)s * PER_SAMPLE + 3 * H + q];
+    } else if ((q -= H) < NO * H) {  // W3[j][i] += go[j] * h2[i]
+        int j = q / H, i = q % H;
+        for (int s = 0; s < batch; ++s) acc += work[(long long)s * PER_SAMPLE + 4 * H + j] * work[(long long)s * PER_SAMPLE + H + i];
+    } else {  // b3
+        q -= NO * H;
+        for (int s = 0; s < batch; ++s) acc += work[(long long)s * PER_SAMPLE + 4 * H + q];
+    }
+    grad[pidx] = acc;
+}
+
+__global__ void k_adam(float *p, const float *g, float *m, float *v, int n, float lr, float b1,
+                       float b2, float eps, float c1, float c2) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float gi = g[i];
+    float mi = b1 * m[i] + (1.f - b1) * gi;
+    float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi; v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// ------------------------------------------------------------------------------------
+// launchers
+// ------------------------------------------------------------------------------------
+static int grid_for(long long n, int block) {
+    long long g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > 65536) g = 65536;
+    return (int)g;
+}
+
+hipError_t launch_fill_i32(int *p, long long n, int v, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+hipError_t launch_fill_u32(unsigned *p, long long n, unsigned v, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+hipError_t launch_validate_csr(const int *row, const int *col, int n, long long m, int *bad,
+                               hipStream_t s) {
+    long long work = (long long)n + 1 > m ? (long long)n + 1 : m;
+    hipLaunchKernelGGL(k_validate_csr, dim3(grid_for(work, 256) > 4096 ? 4096 : grid_for(work, 256)),
+                       dim3(256), 0, s, row, col, n, m, bad);
+    return hipGetLastError();
+}
+hipError_t launch_degree_class(const int *row, int n, unsigned char *cls, int *hist, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_degree_class, dim3(grid_for(n, WG)), dim3(WG), 0, s, row, n, cls, hist);
+    return hipGetLastError();
+}
+hipError_t launch_init_solve(const DevState &st, const int *d_src_slots, int nsrc, int push,
+                             hipStream_t s) {
+    hipError_t e;
+    if ((e = launch_fill_i32(st.dist, st.nslots, st.inf, s)) != hipSuccess) return e;
+    if ((e = launch_fill_u32(st.bm[0], st.nwords, 0u, s)) != hipSuccess) return e;
+    if ((e = launch_fill_u32(st.bm[1], st.nwords, 0u, s)) != hipSuccess) return e;
+    if ((e = launch_fill_i32(st.cnt, 3 * NCLS, 0, s)) != hipSuccess) return e;
+    if ((e = launch_fill_i32((int *)st.ctrl, (long long)(sizeof(Ctrl) / sizeof(int)), 0, s)) != hipSuccess)
+        return e;
+    if (push) hipLaunchKernelGGL(k_init_sources<true>, dim3(1), dim3(256), 0, s, st, d_src_slots, nsrc);
+    else hipLaunchKernelGGL(k_init_sources<false>, dim3(1), dim3(256), 0, s, st, d_src_slots, nsrc);
+    return hipGetLastError();
+}
+hipError_t launch_reached_work(const int *row, const int *dist, int n, int inf,
+                               unsigned long long *out2, hipStream_t s) {
+    hipError_t e = launch_fill_u32((unsigned *)out2, 4, 0u, s);
+    if (e != hipSuccess) return e;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_reached_work, dim3(grid_for(n, WG) > 2048 ? 2048 : grid_for(n, WG)), dim3(WG),
+                       0, s, row, dist, n, inf, out2);
+    return hipGetLastError();
+}
+hipError_t launch_rebase_row(const int *row, int n, int add, int *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_rebase_row, dim3(grid_for((long long)n + 1, 256)), dim3(256), 0, s, row, n, add, out);
+    return hipGetLastError();
+}
+hipError_t launch_fill_lut(int *lut, long long first, long long count, int inst, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_lut, dim3(grid_for(count, 256)), dim3(256), 0, s, lut, first, count, inst);
+    return hipGetLastError();
+}
+hipError_t launch_copy_cls(const unsigned char *src, unsigned char *dst, long long n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_copy_u8, dim3(grid_for(n, 256)), dim3(256), 0, s, src, dst, n);
+    return hipGetLastError();
+}
+
+template <bool B, bool U, bool P>
+static const void *persistent_fn() { return (const void *)&k_persistent<B, U, P>; }
+
+static const void *pick_persistent(bool batch, bool unit, bool push) {
+    if (batch) {
+        if (unit) return push ? persistent_fn<true, true, true>() : persistent_fn<true, true, false>();
+        return push ? persistent_fn<true, false, true>() : persistent_fn<true, false, false>();
+    }
+    if (unit) return push ? persistent_fn<false, true, true>() : persistent_fn<false, true, false>();
+    return push ? persistent_fn<false, false, true>() : persistent_fn<false, false, false>();
+}
+
+int persistent_max_wgs_per_cu(bool batch, bool unit, bool push) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pick_persistent(batch, unit, push), WG, 0) !=
+        hipSuccess)
+        return 0;
+    return nb;
+}
+
+hipError_t launch_persistent(const DevState &st, const SchedDev &sp, bool batch, bool unit, int wgs,
+                             hipStream_t s) {
+    DevState a0 = st;
+    SchedDev a1 = sp;
+    void *args[] = {&a0, &a1};
+    return hipLaunchCooperativeKernel(pick_persistent(batch, unit, sp.push != 0), dim3(wgs), dim3(WG),
+                                      args, 0, s);
+}
+
+hipError_t launch_compact(const DevState &st, int it_off, bool batch, int wgs, hipStream_t s) {
+    (void)batch;
+    long long need = (st.nwords + WG - 1) / WG;
+    int g = (int)(need < wgs ? (need < 1 ? 1 : need) : wgs);
+    hipLaunchKernelGGL(k_compact, dim3(g), dim3(WG), 0, s, st, it_off);
+    return hipGetLastError();
+}
+hipError_t launch_decide(const DevState &st, const SchedDev &sp, int it_off, hipStream_t s) {
+    hipLaunchKernelGGL(k_decide, dim3(1), dim3(WG), 0, s, st, sp, it_off);
+    return hipGetLastError();
+}
+hipError_t launch_expand(const DevState &st, const SchedDev &sp, int it_off, bool batch, bool unit,
+                         int wgs, hipStream_t s) {
+    const bool push = sp.push != 0;
+#define NP_EXP(B, U, P) hipLaunchKernelGGL((k_expand<B, U, P>), dim3(wgs), dim3(WG), 0, s, st, it_off)
+    if (batch) {
+        if (unit) { if (push) NP_EXP(true, true, true); else NP_EXP(true, true, false); }
+        else { if (push) NP_EXP(true, false, true); else NP_EXP(true, false, false); }
+    } else {
+        if (unit) { if (push) NP_EXP(false, true, true); else NP_EXP(false, true, false); }
+        else { if (push) NP_EXP(false, false, true); else NP_EXP(false, false, false); }
+    }
+#undef NP_EXP
+    return hipGetLastError();
+}
+hipError_t launch_advance(const DevState &st, int k, hipStream_t s) {
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, s, st, k);
+    return hipGetLastError();
+}
+hipError_t launch_set_cfg(const DevState &st, int level, int klo, int khi, hipStream_t s) {
+    hipLaunchKernelGGL(k_set_cfg, dim3(1), dim3(64), 0, s, st, level, klo, khi);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_forward(const float *p, const float *x, int batch, float *out, hipStream_t s) {
+    if (batch <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_mlp_forward, dim3((batch + 127) / 128), dim3(128), 0, s, p, x, batch, out);
+    return hipGetLastError();
+}
+size_t mlp_work_floats(int batch) { return (size_t)(batch > 0 ? batch : 1) * PER_SAMPLE; }
+hipError_t launch_mlp_loss_grad(const float *p, const float *x, const int *label, const float *target,
+                                int batch, float lam, float *work, float *grad, float *loss,
+                                hipStream_t s) {
+    if (batch <= 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_mlp_sample_grad, dim3((batch + 127) / 128), dim3(128), 0, s, p, x, label,
+                       target, batch, lam, work);
+    hipLaunchKernelGGL(k_mlp_reduce, dim3((NP_MLP_NPARAM + 1 + 127) / 128), dim3(128), 0, s, x, batch,
+                       work, grad, loss);
+    return hipGetLastError();
+}
+hipError_t launch_adam(float *p, const float *grad, float *m, float *v, int n, float lr, float b1,
+                       float b2, float eps, int step, hipStream_t s) {
+    float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
+    hipLaunchKernelGGL(k_adam, dim3((n + 255) / 256), dim3(256), 0, s, p, grad, m, v, n, lr, b1, b2, eps,
+                       c1, c2);
+    return hipGetLastError();
+}
+
+}  // namespace np

@@ -1,0 +1,237 @@
+"""GPU parity of the SSSP / BFS path: the HIP kernels through the C ABI against the oracle
+and the reference fixtures, bit-exact (int32 distances and levels)."""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, load_golden, random_csr
+
+pytestmark = pytest.mark.gpu
+
+FIXTURES = sorted(os.path.basename(p) for p in glob.glob(os.path.join(GOLDEN, "*.npz")))
+
+# (level, log2_t_lo, log2_t_hi): the plain schedules of the reference at each level and a
+# few three-bin configurations that put work in every scope
+SCHEDS = [(0, 6, 6), (1, 6, 6), (2, 6, 6), (1, 3, 8), (2, 2, 4), (2, 0, 0), (0, 30, 30), (1, 4, 12)]
+DRIVES = [(0, 0), (0, 1), (1, 0), (1, 1), (2, 0), (2, 1)]  # (drive, frontier)
+
+
+def _sched(npa, lv, lo, hi, drive=0, frontier=0, **kw):
+    return npa.Sched(level=lv, log2_t_lo=lo, log2_t_hi=hi, drive=drive, frontier=frontier, **kw)
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_fixtures_all_levels(npa, ctx, name):
+    g = load_golden(name)
+    G = npa.Graph.from_csr(ctx, g["row"], g["col"], g["w"])
+    for k, s in enumerate(g["sources"]):
+        for lv, lo, hi in SCHEDS:
+            d, st = ctx.sssp(G, s, _sched(npa, lv, lo, hi))
+            assert np.array_equal(d, g["dist"][k]), (name, s, lv, lo, hi)
+        lev, _ = ctx.bfs(G, s)
+        assert np.array_equal(lev, g["level"][k]), (name, s)
+    G.free()
+
+
+@pytest.mark.parametrize("drive,frontier", DRIVES)
+def test_drives_and_frontiers(npa, ctx, orc, drive, frontier):
+    rng = np.random.default_rng(100 + 10 * drive + frontier)
+    row, col, w = random_csr(rng, 20000, 200000, 1, 100)
+    G = npa.Graph.from_csr(ctx, row, col, w)
+    for s in (0, 19999):
+        ref, _ = orc.dijkstra(row, col, w, s)
+        for lv, lo, hi in SCHEDS[:4]:
+            d, st = ctx.sssp(G, s, _sched(npa, lv, lo, hi, drive, frontier))
+            assert np.array_equal(d, ref), (drive, frontier, lv, lo, hi)
+            assert st.iterations > 0
+        lev, _ = ctx.bfs(G, s, _sched(npa, 2, 6, 6, drive, frontier))
+        assert np.array_equal(lev, orc.bfs_levels(row, col, s))
+    G.free()
+
+
+def test_er_64k_config(npa, ctx, orc):
+    """Directed ER n=65,536, m=1,048,576, weights U[1,100], source 0, block level."""
+    G = npa.Graph.er(ctx, 65536, 1048576, 1, 100, seed=1)
+    row, col, w = G.download()
+    hrow, hcol, hw = npa.gen_er_host(65536, 1048576, 1, 100, seed=1)
+    assert np.array_equal(row, hrow) and np.array_equal(col, hcol) and np.array_equal(w, hw)
+    ref, _ = orc.dijkstra(row, col, w, 0)
+    for lv in (0, 1, 2):
+        d, st = ctx.sssp(G, 0, npa.Sched(level=lv))
+        assert np.array_equal(d, ref), lv
+    G.free()
+
+
+def test_edge_cases(npa, ctx, orc):
+    rng = np.random.default_rng(7)
+    cases = [
+        (np.array([0, 0], np.int32), np.zeros(0, np.int32), np.zeros(0, np.int32), 0),      # n=1, m=0
+        (np.array([0, 0, 1], np.int32), np.array([0], np.int32), np.array([5], np.int32), 0),  # isolated source
+    ]
+    row, col, w = random_csr(rng, 3000, 9000, 0, 3)      # zero weights, self loops, duplicates
+    cases.append((row, col, w, 1))
+    n = 200001                                            # one hub with 200000 out-edges
+    hrow = np.zeros(n + 1, np.int32); hrow[1:] = n - 1
+    hcol = np.arange(1, n, dtype=np.int32)
+    hw = (np.arange(n - 1) % 255 + 1).astype(np.int32)
+    cases.append((hrow, hcol, hw, 0))
+    for row, col, w, s in cases:
+        G = npa.Graph.from_csr(ctx, row, col, w)
+        ref, _ = orc.dijkstra(row, col, w, s)
+        for lv, lo, hi in SCHEDS:
+            for fr in (0, 1):
+                d, _ = ctx.sssp(G, s, _sched(npa, lv, lo, hi, 0, fr))
+                assert np.array_equal(d, ref), (len(row) - 1, lv, lo, hi, fr)
+        G.free()
+
+
+def test_bad_arguments(npa, ctx):
+    G = npa.Graph.from_csr(ctx, [0, 1, 1], [1], [3])
+    with pytest.raises(npa.NpError) as e:
+        ctx.sssp(G, 2)
+    assert e.value.code == -1
+    with pytest.raises(npa.NpError):
+        ctx.sssp(G, 0, npa.Sched(level=3))
+    with pytest.raises(npa.NpError):
+        npa.Graph.from_csr(ctx, [0, 2, 1], [1], [3])
+    G.free()
+
+
+@pytest.mark.parametrize("gen", ["rmat", "er", "grid"])
+def test_device_generators_equal_host(npa, ctx, gen):
+    if gen == "rmat":
+        G = npa.Graph.rmat(ctx, 12, 16, wmin=1, wmax=255, seed=7)
+        h = npa.gen_rmat_host(12, 16, wmin=1, wmax=255, seed=7)
+    elif gen == "er":
+        G = npa.Graph.er(ctx, 5000, 40000, 1, 100, seed=3)
+        h = npa.gen_er_host(5000, 40000, 1, 100, seed=3)
+    else:
+        G = npa.Graph.grid(ctx, 37, 53, None, 1, 1000, seed=5)
+        h = npa.gen_grid_host(37, 53, None, 1, 1000, seed=5)
+    d = G.download()
+    for a, b in zip(d, h):
+        assert np.array_equal(a, b)
+    G.free()
+
+
+def test_rmat_power_law_all_levels(npa, ctx, orc):
+    G = npa.Graph.rmat(ctx, 16, 16, wmin=1, wmax=255, seed=2)
+    row, col, w = G.download()
+    srcs = npa.pick_sources(row, 3, 2)
+    for s in srcs:
+        ref, _ = orc.dijkstra(row, col, w, s)
+        for lv, lo, hi in SCHEDS:
+            d, _ = ctx.sssp(G, s, _sched(npa, lv, lo, hi))
+            assert np.array_equal(d, ref), (s, lv, lo, hi)
+    G.free()
+
+
+def test_grid_high_diameter(npa, ctx, orc):
+    G = npa.Graph.grid(ctx, 96, 80, None, 1, 1000, seed=5)
+    row, col, w = G.download()
+    for s in (0, 3000):
+        ref, _ = orc.dijkstra(row, col, w, s)
+        for drive, fr in DRIVES:
+            d, st = ctx.sssp(G, s, _sched(npa, 1, 6, 6, drive, fr))
+            assert np.array_equal(d, ref), (drive, fr)
+    G.free()
+
+
+def test_batch_sources_and_graphs(npa, ctx, orc):
+    G = npa.Graph.rmat(ctx, 13, 16, wmin=1, wmax=255, seed=4)
+    row, col, w = G.download()
+    srcs = list(npa.pick_sources(row, 6, 4))
+    small = [npa.Graph.rmat(ctx, sc, 16, wmin=1, wmax=255, seed=1000 + i) for i, sc in enumerate([9, 11, 10, 12])]
+    graphs = [G] * len(srcs) + small
+    ssrc = [int(npa.pick_sources(g.download()[0], 1, 1000 + i)[0]) for i, g in enumerate(small)]
+    allsrc = srcs + ssrc
+    B = npa.Batch(ctx, graphs, allsrc)
+    for drive, fr in DRIVES:
+        st = B.solve(_sched(npa, 1, 6, 6, drive, fr))
+        for i, (g, s) in enumerate(zip(graphs, allsrc)):
+            r, c, ww = g.download()
+            ref, _ = orc.dijkstra(r, c, ww, s)
+            assert np.array_equal(B.get(i), ref), (drive, fr, i)
+    B.free()
+    Bu = npa.Batch(ctx, graphs, allsrc, unit=True)
+    Bu.solve(_sched(npa, 2, 6, 6))
+    for i, (g, s) in enumerate(zip(graphs, allsrc)):
+        r, c, _ = g.download()
+        assert np.array_equal(Bu.get(i), orc.bfs_levels(r, c, s)), i
+    Bu.free()
+    for g in small:
+        g.free()
+    G.free()
+
+
+def test_reached_work_and_trace(npa, ctx, orc):
+    G = npa.Graph.rmat(ctx, 14, 16, wmin=1, wmax=255, seed=9)
+    row, col, w = G.download()
+    s = int(npa.pick_sources(row, 1, 9)[0])
+    d_dist = npa.DeviceBuffer(ctx, G.n)
+    st = ctx.sssp_device(G, s, d_dist.ptr, npa.Sched(level=2, trace=1))
+    ref, _ = orc.dijkstra(row, col, w, s)
+    assert np.array_equal(d_dist.to_host(), ref)
+    assert ctx.reached_work(G, d_dist.ptr) == orc.reached_work(row, ref)
+    d_dist.free()
+    tr = ctx.trace()
+    assert len(tr) == st.iterations
+    assert sum(sum(r.count[1:]) for r in tr) == st.frontier_total
+    assert tr[0].count[1:].__len__() == 31 and sum(tr[0].count[1:]) == 1
+
+
+def test_learned_schedule_is_exact_and_falls_back(npa, ctx, orc):
+    G = npa.Graph.rmat(ctx, 14, 16, wmin=1, wmax=255, seed=11)
+    row, col, w = G.download()
+    s = int(npa.pick_sources(row, 1, 11)[0])
+    ref, _ = orc.dijkstra(row, col, w, s)
+    rng = np.random.default_rng(1)
+    for scale in (0.5, 5.0):
+        ctx.model_set((rng.standard_normal(npa.MLP_NPARAM) * scale).astype(np.float32))
+        d, _ = ctx.sssp(G, s, npa.Sched(level=npa.LEVEL_LEARNED, trace=1))
+        assert np.array_equal(d, ref)
+    # weights that overflow fp32 in the forward pass: the fixed heuristic must take over
+    ctx.model_set(np.full(npa.MLP_NPARAM, 3e37, np.float32))
+    d, st = ctx.sssp(G, s, npa.Sched(level=npa.LEVEL_LEARNED, log2_t_lo=6, log2_t_hi=6, trace=1))
+    assert np.array_equal(d, ref)
+    tr = ctx.trace()
+    assert all(r.level == 1 and r.log2_t_lo == 6 for r in tr)
+    G.free()
+
+
+def test_mlp_kernels_vs_oracle(npa, ctx, orc):
+    rng = np.random.default_rng(0)
+    params = (rng.standard_normal(npa.MLP_NPARAM) * 0.3).astype(np.float32)
+    x = rng.random((300, 10)).astype(np.float32)
+    label = rng.integers(0, 3, 300).astype(np.int32)
+    target = (rng.random((300, 2)) * 10).astype(np.float32)
+    ctx.model_set(params)
+    np.testing.assert_allclose(ctx.model_forward(x), orc.mlp_forward(params, x), rtol=1e-5, atol=1e-5)
+    loss, grad = ctx.model_loss_grad(x, label, target, 0.1)
+    loss_o, grad_o = orc.mlp_loss_grad(params, x, label, target, 0.1)
+    assert abs(loss - loss_o) <= 1e-4 * max(1, abs(loss_o))
+    np.testing.assert_allclose(grad, grad_o, rtol=1e-3, atol=1e-6)
+    losses = ctx.model_train(x, label, target, 200, lr=1e-2, lam=0.1)
+    assert losses[-1] < losses[0]
+
+
+def test_stepping_reaches_exact_distances(npa, ctx, orc):
+    G = npa.Graph.rmat(ctx, 13, 16, wmin=1, wmax=255, seed=12)
+    row, col, w = G.download()
+    s = int(npa.pick_sources(row, 1, 12)[0])
+    ctx.step_begin(G, s)
+    k = 0
+    while True:
+        cnt, done = ctx.step_frontier()
+        if done:
+            break
+        for lv in (0, 1, 2):
+            assert ctx.step_try(npa.Sched(level=lv), 1) >= 0
+        ctx.step_advance(npa.Sched(level=k % 3))
+        k += 1
+    d = ctx.step_end(G.n)
+    ref, _ = orc.dijkstra(row, col, w, s)
+    assert np.array_equal(d, ref)
+    G.free()
