@@ -86,8 +86,19 @@ typedef struct np_sched {
     int32_t graph_chunk; /* NP_DRIVE_GRAPH: iterations per captured chunk (0 = 16) */
     int32_t max_iters;   /* 0 = no cap beyond n+1 */
     int32_t trace;       /* nonzero: record per-iteration trace (np_trace_*) */
-    int32_t reserved[7];
+    int32_t options;     /* NP_OPT_* bits */
+    int32_t delta;       /* scan frontier: priority window.  0 = every flagged vertex is
+                            expanded (the reference's unordered frontier); > 0 = only
+                            vertices with dist < T are expanded, T rising to
+                            min(pending) + delta when none is left.  Changes the order of
+                            the relaxations only, never a distance. */
+    int32_t reserved[5];
 } np_sched;
+
+/* np_sched.options: the check "dist[v] > alt" before the atomicMin reads dist through
+ * the caches (may see an older, larger value: only more atomics, never a wrong result)
+ * instead of a device-coherent load. */
+#define NP_OPT_CACHED_FILTER 1
 
 /* Fills the plain fixed-threshold schedule of the reference at `level` (0,1,2), the
  * persistent drive and the scan frontier. */

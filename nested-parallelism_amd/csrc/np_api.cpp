@@ -212,7 +212,7 @@ static int validate_sched(const np_sched *in, np_sched *out) {
     }
     if (out->drive < 0 || out->drive > 2) { set_error("bad drive"); return NP_ERR_ARG; }
     if (out->frontier < 0 || out->frontier > 1) { set_error("bad frontier mode"); return NP_ERR_ARG; }
-    if (out->wgs_per_cu < 0 || out->graph_chunk < 0 || out->max_iters < 0) {
+    if (out->wgs_per_cu < 0 || out->graph_chunk < 0 || out->max_iters < 0 || out->delta < 0) {
         set_error("negative schedule field"); return NP_ERR_ARG;
     }
     return NP_OK;
@@ -225,6 +225,7 @@ static SchedDev make_schedd(np_ctx *c, const np_sched &s, int n_feat) {
     d.klo = s.log2_t_lo;
     d.khi = s.log2_t_hi;
     d.push = s.frontier == NP_FRONTIER_PUSH;
+    d.delta = d.push ? 0 : s.delta;
     // With non-negative weights the frontier empties within n+1 expansions (every vertex
     // is final once all shortest paths of <= k edges are), so n+2 only stops inputs
     // with negative cycles.
@@ -358,9 +359,10 @@ static int solve_single(np_ctx *c, np_graph *g, int src, const np_sched *sched, 
     int *dist = d_dist_user ? d_dist_user : s.dist;
     const int inf = unit ? UNDEFINED : INF;
     DevState st = make_state(c, s, g->row, g->col, unit ? nullptr : g->w, dist, inf);
+    st.cached_filter = (sch.options & NP_OPT_CACHED_FILTER) ? 1 : 0;
     SchedDev sd = make_schedd(c, sch, g->n);
     HIPCHK(hipMemcpyAsync(s.src_slots, &src, sizeof(int), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(launch_init_solve(st, s.src_slots, 1, sd.push, c->stream));
+    HIPCHK(launch_init_solve(st, s.src_slots, 1, sd.push, sd.delta, c->stream));
     rc = run_drive(c, s, st, sch, sd, false, unit, stats);
     if (rc) return rc;
     if (h_dist) {
@@ -722,10 +724,11 @@ int np_batch_solve(np_ctx *c, np_batch *b, const np_sched *sched, np_stats *stat
     Space &s = b->sp;
     const bool unit = b->unit != 0;
     DevState st = make_state(c, s, b->row, b->col, unit ? nullptr : b->w, s.dist, unit ? UNDEFINED : INF);
+    st.cached_filter = (sch.options & NP_OPT_CACHED_FILTER) ? 1 : 0;
     int nmax = 1;
     for (int i = 0; i < b->count; ++i) nmax = std::max(nmax, b->nverts[i]);
     SchedDev sd = make_schedd(c, sch, nmax);
-    HIPCHK(launch_init_solve(st, s.src_slots, b->count, sd.push, c->stream));
+    HIPCHK(launch_init_solve(st, s.src_slots, b->count, sd.push, sd.delta, c->stream));
     return run_drive(c, s, st, sch, sd, true, unit, stats);
 }
 
@@ -895,7 +898,7 @@ int np_step_begin(np_ctx *c, np_graph *g, int32_t src, np_stats *stats) {
     DevState st; SchedDev sd;
     if ((rc = step_state(c, st, sd))) return rc;
     HIPCHK(hipMemcpyAsync(g->sp.src_slots, &src, sizeof(int), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(launch_init_solve(st, g->sp.src_slots, 1, 0, c->stream));
+    HIPCHK(launch_init_solve(st, g->sp.src_slots, 1, 0, 0, c->stream));
     HIPCHK(launch_set_cfg(st, 1, 6, 6, c->stream));
     if (stats) std::memset(stats, 0, sizeof(*stats));
     return step_compact(c);

@@ -8,15 +8,24 @@
 
 namespace np {
 
+constexpr int BAR_LEAVES = 32;
+struct alignas(128) BarLeaf { unsigned long long n; unsigned long long pad[15]; };
+
 // Device-side control block of one solve.
 struct Ctrl {
-    unsigned long long bar;   // grid barrier: cumulative arrivals
+    BarLeaf leaf[BAR_LEAVES]; // grid barrier: cumulative arrivals per leaf (own cache lines)
+    unsigned long long bar;   // grid barrier: cumulative leaf completions
     int error;                // NP_ERR_* raised by a kernel (0 = none)
     int done;                 // frontier emptied
     int iters;                // expansions performed
     int it_base;              // graph drive: iteration index of the captured chunk's first slot
     long long frontier_total; // sum of frontier sizes
     int cfg_level, cfg_klo, cfg_khi, cfg_pad;  // schedule decided for the next expansion
+    int thresh;               // priority window: vertices with dist < thresh are expanded
+    int pass;                 // compaction passes so far (host-loop / graph drives)
+    int skip;                 // host-loop / graph drives: this slot only raised the window
+    int defer_cnt[3];         // per compaction pass (rotating): flagged vertices kept back
+    int defer_min[3];         // ... and the least distance among them
 };
 
 // Everything an expansion / compaction touches.  Indices are "slots": the vertex ids of
@@ -36,6 +45,7 @@ struct DevState {
     long long nslots;
     long long nwords;         // bitmap words
     int inf;                  // unreached value written at init
+    int cached_filter;        // NP_OPT_CACHED_FILTER
     Ctrl *ctrl;
 };
 
@@ -43,6 +53,7 @@ struct SchedDev {
     int learned;            // evaluate the MLP each iteration
     int level, klo, khi;    // fixed schedule (the fallback when the MLP is unusable)
     int push;               // NP_FRONTIER_PUSH
+    int delta;              // priority window (0 = off)
     int max_iters;
     int trace_cap;
     int n_feat;             // vertex count used by the features
@@ -57,7 +68,7 @@ hipError_t launch_fill_u32(unsigned *p, long long n, unsigned v, hipStream_t s);
 hipError_t launch_degree_class(const int *row, int n, unsigned char *cls, int *hist, hipStream_t s);
 hipError_t launch_validate_csr(const int *row, const int *col, int n, long long m, int *bad,
                                hipStream_t s);
-hipError_t launch_init_solve(const DevState &st, const int *d_src_slots, int nsrc, int push,
+hipError_t launch_init_solve(const DevState &st, const int *d_src_slots, int nsrc, int push, int delta,
                              hipStream_t s);
 hipError_t launch_reached_work(const int *row, const int *dist, int n, int inf,
                                unsigned long long *out2, hipStream_t s);

@@ -75,8 +75,10 @@ __device__ __forceinline__ void decode(const DevState &st, int slot, int &B, int
 
 // Grid-wide barrier of a cooperative launch.  Every wave first waits for its own
 // outstanding memory operations (s_waitcnt 0: its coherent stores and atomics have
-// completed), the workgroup meets at s_barrier, then one lane arrives on a cumulative
-// 64-bit counter (no reset) and spins with relaxed coherent loads.  The spin is
+// completed), the workgroup meets at s_barrier, then one lane arrives.  Arrivals form a
+// two-level tree of cumulative 64-bit counters (no reset): workgroup b counts on leaf
+// b % 32 (each on its own cache line), the last arrival of a leaf counts on the root,
+// and every workgroup spins on the root with relaxed coherent loads.  The spin is
 // bounded: on expiry the block raises NP_ERR_TIMEOUT and every block leaves its loop at
 // the next barrier, so the grid always drains.
 __device__ bool grid_sync(Ctrl *ctrl, unsigned nblocks, unsigned long long &gen) {
@@ -85,9 +87,15 @@ __device__ bool grid_sync(Ctrl *ctrl, unsigned nblocks, unsigned long long &gen)
     __syncthreads();
     if (threadIdx.x == 0) {
         gen += 1;
-        const unsigned long long target = gen * (unsigned long long)nblocks;
-        unsigned long long arrived =
-            __hip_atomic_fetch_add(&ctrl->bar, 1ull, __ATOMIC_RELAXED, NP_AGENT) + 1ull;
+        const unsigned nleaves = nblocks < (unsigned)BAR_LEAVES ? nblocks : (unsigned)BAR_LEAVES;
+        const unsigned l = blockIdx.x % nleaves;
+        const unsigned long long lsize = nblocks / nleaves + (l < nblocks % nleaves ? 1u : 0u);
+        unsigned long long la =
+            __hip_atomic_fetch_add(&ctrl->leaf[l].n, 1ull, __ATOMIC_RELAXED, NP_AGENT) + 1ull;
+        const unsigned long long target = gen * (unsigned long long)nleaves;
+        unsigned long long arrived = 0;
+        if (la == gen * lsize)
+            arrived = __hip_atomic_fetch_add(&ctrl->bar, 1ull, __ATOMIC_RELAXED, NP_AGENT) + 1ull;
         int ok = 1;
         if (arrived < target) {
             unsigned long long spins = 0;
@@ -180,7 +188,8 @@ __device__ __forceinline__ void relax_range(const DevState &st, const PushCtx &p
             alt[k] = du + (UNIT ? 1 : st.w[ec]);
         }
 #pragma unroll
-        for (int k = 0; k < UNR; ++k) cur[k] = ok[k] ? ld_coherent(st.dist + tv[k]) : 0;
+        for (int k = 0; k < UNR; ++k)
+            cur[k] = ok[k] ? (st.cached_filter ? st.dist[tv[k]] : ld_coherent(st.dist + tv[k])) : 0;
 #pragma unroll
         for (int k = 0; k < UNR; ++k) {
             bool want = false;
@@ -237,7 +246,8 @@ __device__ __forceinline__ void lane_tile(const DevState &st, const PushCtx &pc,
             alt[k] = ou + (UNIT ? 1 : st.w[ec]);
         }
 #pragma unroll
-        for (int k = 0; k < UNR; ++k) cur[k] = ok[k] ? ld_coherent(st.dist + tv[k]) : 0;
+        for (int k = 0; k < UNR; ++k)
+            cur[k] = ok[k] ? (st.cached_filter ? st.dist[tv[k]] : ld_coherent(st.dist + tv[k])) : 0;
 #pragma unroll
         for (int k = 0; k < UNR; ++k) {
             bool want = false;
@@ -347,20 +357,36 @@ __device__ void expansion(const DevState &st, const PushCtx &pc, const int *q, I
 // ------------------------------------------------------------------------------------
 // compaction (scan frontier): update bitmap -> class-segmented queue
 // ------------------------------------------------------------------------------------
+// With a priority window (T < INT_MAX) only flagged vertices with dist < T are queued and
+// unflagged; the others keep their flag, and the pass adds how many were kept and the
+// least distance among them to defer_cnt / defer_min.  T = INT_MAX queues every flag.
+constexpr int NO_WINDOW = 0x7FFFFFFF;
+
 __device__ __forceinline__ void compaction_impl(unsigned *bm, const unsigned char *cls, const int *seg_off,
-                                                long long nwords, int *cnt_out, int *q_out) {
+                                                long long nwords, int *cnt_out, int *q_out, const int *dist,
+                                                int T, int *defer_cnt, int *defer_min) {
     __shared__ int s_lc[NCLS], s_lb[NCLS];
+    __shared__ int s_dc, s_dm;
+    if (threadIdx.x == 0) { s_dc = 0; s_dm = NO_WINDOW; }
+    int my_dc = 0, my_dm = NO_WINDOW;
     for (long long wb = (long long)blockIdx.x * WG; wb < nwords; wb += (long long)gridDim.x * WG) {
         if (threadIdx.x < NCLS) s_lc[threadIdx.x] = 0;
         __syncthreads();
         const long long wi = wb + threadIdx.x;
         const unsigned word0 = wi < nwords ? ld_coherent_u(bm + wi) : 0u;
-        unsigned word = word0;
+        unsigned word = word0, take = 0;
         while (word) {
             int b = __ffs(word) - 1;
             word &= word - 1;
-            int c = cls[wi * 32 + b];
-            if (c) atomicAdd(&s_lc[c], 1);
+            const long long slot = wi * 32 + b;
+            int c = cls[slot];
+            if (!c) continue;
+            if (T != NO_WINDOW) {
+                int d = ld_coherent(dist + slot);
+                if (d >= T) { my_dc++; my_dm = d < my_dm ? d : my_dm; continue; }
+            }
+            take |= 1u << b;
+            atomicAdd(&s_lc[c], 1);
         }
         __syncthreads();
         if (threadIdx.x < NCLS) {
@@ -369,19 +395,28 @@ __device__ __forceinline__ void compaction_impl(unsigned *bm, const unsigned cha
             s_lc[threadIdx.x] = 0;
         }
         __syncthreads();
-        word = word0;
+        word = take;
         while (word) {
             int b = __ffs(word) - 1;
             word &= word - 1;
             long long slot = wi * 32 + b;
             int c = cls[slot];
-            if (c) {
-                int p = atomicAdd(&s_lc[c], 1);
-                st_coherent(q_out + seg_off[c] + s_lb[c] + p, (int)slot);
-            }
+            int p = atomicAdd(&s_lc[c], 1);
+            st_coherent(q_out + seg_off[c] + s_lb[c] + p, (int)slot);
         }
-        if (word0) st_coherent_u(bm + wi, 0u);
+        if (take) st_coherent_u(bm + wi, word0 & ~take);
         __syncthreads();
+    }
+    if (T != NO_WINDOW) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            my_dc += __shfl_xor(my_dc, o);
+            int t = __shfl_xor(my_dm, o);
+            my_dm = t < my_dm ? t : my_dm;
+        }
+        if (lane_id() == 0 && my_dc) { atomicAdd(&s_dc, my_dc); atomicMin(&s_dm, my_dm); }
+        __syncthreads();
+        if (threadIdx.x == 0 && s_dc) { atomicAdd(defer_cnt, s_dc); atomicMin(defer_min, s_dm); }
     }
 }
 
@@ -452,6 +487,9 @@ __global__ __launch_bounds__(WG) void k_persistent(DevState st, SchedDev sp) {
     unsigned long long gen = 0;
     const unsigned nb = gridDim.x;
     long long t_prev = 0;
+    // priority window: every workgroup computes the same sequence of thresholds
+    int T = (!PUSH && sp.delta > 0) ? sp.delta : NO_WINDOW;
+    int pass = 0;
     for (int it = 0;; ++it) {
         int *cnt_cur = st.cnt + (it % 3) * NCLS;
         int *cnt_zero = st.cnt + ((it + 2) % 3) * NCLS;
@@ -462,21 +500,35 @@ __global__ __launch_bounds__(WG) void k_persistent(DevState st, SchedDev sp) {
             t_prev = now;
         }
         if (blockIdx.x == 0 && threadIdx.x < NCLS) st_coherent(cnt_zero + threadIdx.x, 0);
-        if (!PUSH) {
-            compaction_impl(st.bm[0], st.cls, st.seg_off, st.nwords, cnt_cur, q_cur);
-            if (!grid_sync(ctrl, nb, gen)) return;
+        for (;;) {
+            int *dc = ctrl->defer_cnt + pass % 3, *dm = ctrl->defer_min + pass % 3;
+            if (!PUSH) {
+                // the statistics slot of the next pass was last read before the previous barrier
+                if (blockIdx.x == 0 && threadIdx.x == 0) {
+                    st_coherent(ctrl->defer_cnt + (pass + 1) % 3, 0);
+                    st_coherent(ctrl->defer_min + (pass + 1) % 3, NO_WINDOW);
+                }
+                compaction_impl(st.bm[0], st.cls, st.seg_off, st.nwords, cnt_cur, q_cur, st.dist, T, dc, dm);
+                if (!grid_sync(ctrl, nb, gen)) return;
+                pass++;
+            }
+            if (threadIdx.x < NCLS) {
+                tb.cnt[threadIdx.x] = ld_coherent(cnt_cur + threadIdx.x);
+                tb.seg[threadIdx.x] = st.seg_off[threadIdx.x];
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                long long tot = 0;
+                for (int c = 1; c < NCLS; ++c) tot += tb.cnt[c];
+                s_total = tot > 0x7FFFFFFF ? 0x7FFFFFFF : (int)tot;
+            }
+            __syncthreads();
+            if (s_total > 0 || T == NO_WINDOW) break;
+            // nothing below the window: raise it to the least pending distance + delta
+            if (ld_coherent(dc) == 0) break;
+            const int mn = ld_coherent(dm);
+            T = mn > NO_WINDOW - sp.delta ? NO_WINDOW : mn + sp.delta;
         }
-        if (threadIdx.x < NCLS) {
-            tb.cnt[threadIdx.x] = ld_coherent(cnt_cur + threadIdx.x);
-            tb.seg[threadIdx.x] = st.seg_off[threadIdx.x];
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            long long tot = 0;
-            for (int c = 1; c < NCLS; ++c) tot += tb.cnt[c];
-            s_total = tot > 0x7FFFFFFF ? 0x7FFFFFFF : (int)tot;
-        }
-        __syncthreads();
         const int total = s_total;
         if (total == 0 || (sp.max_iters > 0 && it >= sp.max_iters)) {
             if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -510,8 +562,15 @@ __global__ __launch_bounds__(WG) void k_compact(DevState st, int it_off) {
     Ctrl *ctrl = st.ctrl;
     if (ld_coherent(&ctrl->done) || ld_coherent(&ctrl->error)) return;
     const int it = ld_coherent(&ctrl->it_base) + it_off;
+    const int pass = ld_coherent(&ctrl->pass);
+    const int T = ld_coherent(&ctrl->thresh);
     if (blockIdx.x == 0 && threadIdx.x < NCLS) st.cnt[((it + 2) % 3) * NCLS + threadIdx.x] = 0;
-    compaction_impl(st.bm[0], st.cls, st.seg_off, st.nwords, st.cnt + (it % 3) * NCLS, st.queue[it & 1]);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctrl->defer_cnt[(pass + 1) % 3] = 0;
+        ctrl->defer_min[(pass + 1) % 3] = NO_WINDOW;
+    }
+    compaction_impl(st.bm[0], st.cls, st.seg_off, st.nwords, st.cnt + (it % 3) * NCLS, st.queue[it & 1],
+                    st.dist, T, ctrl->defer_cnt + pass % 3, ctrl->defer_min + pass % 3);
 }
 
 __global__ __launch_bounds__(WG) void k_decide(DevState st, SchedDev sp, int it_off) {
@@ -523,15 +582,28 @@ __global__ __launch_bounds__(WG) void k_decide(DevState st, SchedDev sp, int it_
     __syncthreads();
     long long tot = 0;
     for (int c = 1; c < NCLS; ++c) tot += s_cnt[c];
-    if (tot == 0 || (sp.max_iters > 0 && it >= sp.max_iters)) {
+    const int pass = ld_coherent(&ctrl->pass);  // the compaction of this slot
+    if (tot == 0 && !sp.push) {
+        const int T = ld_coherent(&ctrl->thresh);
+        if (T != NO_WINDOW && ld_coherent(ctrl->defer_cnt + pass % 3) > 0) {
+            // nothing below the window: raise it; this slot's expansion is skipped
+            if (threadIdx.x == 0) {
+                const int mn = ld_coherent(ctrl->defer_min + pass % 3);
+                ctrl->thresh = mn > NO_WINDOW - sp.delta ? NO_WINDOW : mn + sp.delta;
+                ctrl->skip = 1;
+                ctrl->pass = pass + 1;
+            }
+            return;
+        }
+    }
+    if (tot == 0 || (sp.max_iters > 0 && ctrl->iters >= sp.max_iters)) {
         if (threadIdx.x == 0) {
-            ctrl->iters = it;
             if (tot == 0) ctrl->done = 1;
             else ctrl->error = NP_ERR_NOT_CONVERGED;
         }
         return;
     }
-    Cfg cfg = decide(sp, s_cnt, it);
+    Cfg cfg = decide(sp, s_cnt, ctrl->iters);
     if (threadIdx.x == 0) {
         if (ctrl->cfg_pad == 0) {  // cfg_pad != 0: a forced schedule (stepping) is kept
             ctrl->cfg_level = cfg.level;
@@ -541,8 +613,10 @@ __global__ __launch_bounds__(WG) void k_decide(DevState st, SchedDev sp, int it_
             cfg = make_cfg(ctrl->cfg_level, ctrl->cfg_klo, ctrl->cfg_khi);
         }
         ctrl->frontier_total += tot;
-        ctrl->iters = it + 1;
-        record_trace(sp, it, cfg, s_cnt, (long long)wall_clock64());
+        record_trace(sp, ctrl->iters, cfg, s_cnt, (long long)wall_clock64());
+        ctrl->iters += 1;
+        ctrl->skip = 0;
+        if (!sp.push) ctrl->pass = pass + 1;
     }
 }
 
@@ -550,7 +624,7 @@ template <bool BATCH, bool UNIT, bool PUSH>
 __global__ __launch_bounds__(WG) void k_expand(DevState st, int it_off) {
     __shared__ ItemTables tb;
     Ctrl *ctrl = st.ctrl;
-    if (ld_coherent(&ctrl->done) || ld_coherent(&ctrl->error)) return;
+    if (ld_coherent(&ctrl->done) || ld_coherent(&ctrl->error) || ld_coherent(&ctrl->skip)) return;
     const int it = ld_coherent(&ctrl->it_base) + it_off;
     if (threadIdx.x < NCLS) {
         tb.cnt[threadIdx.x] = ld_coherent(st.cnt + (it % 3) * NCLS + threadIdx.x);
@@ -632,8 +706,12 @@ __global__ void k_validate_csr(const int *row, const int *col, int n, long long 
 }
 
 template <bool PUSH>
-__global__ void k_init_sources(DevState st, const int *srcs, int nsrc) {
+__global__ void k_init_sources(DevState st, const int *srcs, int nsrc, int T0) {
     if (blockIdx.x != 0) return;
+    if (threadIdx.x == 0) {
+        st.ctrl->thresh = T0;
+        for (int k = 0; k < 3; ++k) st.ctrl->defer_min[k] = NO_WINDOW;
+    }
     for (int i = threadIdx.x; i < nsrc; i += blockDim.x) {
         int slot = srcs[i];
         st.dist[slot] = 0;
@@ -835,8 +913,9 @@ hipError_t launch_degree_class(const int *row, int n, unsigned char *cls, int *h
     hipLaunchKernelGGL(k_degree_class, dim3(grid_for(n, WG)), dim3(WG), 0, s, row, n, cls, hist);
     return hipGetLastError();
 }
-hipError_t launch_init_solve(const DevState &st, const int *d_src_slots, int nsrc, int push,
+hipError_t launch_init_solve(const DevState &st, const int *d_src_slots, int nsrc, int push, int delta,
                              hipStream_t s) {
+    const int T0 = (!push && delta > 0) ? delta : NO_WINDOW;
     hipError_t e;
     if ((e = launch_fill_i32(st.dist, st.nslots, st.inf, s)) != hipSuccess) return e;
     if ((e = launch_fill_u32(st.bm[0], st.nwords, 0u, s)) != hipSuccess) return e;
@@ -844,8 +923,8 @@ hipError_t launch_init_solve(const DevState &st, const int *d_src_slots, int nsr
     if ((e = launch_fill_i32(st.cnt, 3 * NCLS, 0, s)) != hipSuccess) return e;
     if ((e = launch_fill_i32((int *)st.ctrl, (long long)(sizeof(Ctrl) / sizeof(int)), 0, s)) != hipSuccess)
         return e;
-    if (push) hipLaunchKernelGGL(k_init_sources<true>, dim3(1), dim3(256), 0, s, st, d_src_slots, nsrc);
-    else hipLaunchKernelGGL(k_init_sources<false>, dim3(1), dim3(256), 0, s, st, d_src_slots, nsrc);
+    if (push) hipLaunchKernelGGL(k_init_sources<true>, dim3(1), dim3(256), 0, s, st, d_src_slots, nsrc, T0);
+    else hipLaunchKernelGGL(k_init_sources<false>, dim3(1), dim3(256), 0, s, st, d_src_slots, nsrc, T0);
     return hipGetLastError();
 }
 hipError_t launch_reached_work(const int *row, const int *dist, int n, int inf,

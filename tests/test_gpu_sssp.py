@@ -51,6 +51,33 @@ def test_drives_and_frontiers(npa, ctx, orc, drive, frontier):
     G.free()
 
 
+@pytest.mark.parametrize("drive", [0, 1, 2])
+def test_priority_window(npa, ctx, orc, drive):
+    """The priority window changes only the order of relaxations: distances stay exact."""
+    G = npa.Graph.rmat(ctx, 14, 16, wmin=1, wmax=255, seed=21)
+    row, col, w = G.download()
+    srcs = npa.pick_sources(row, 2, 21)
+    for s in srcs:
+        ref, _ = orc.dijkstra(row, col, w, s)
+        lev_ref = orc.bfs_levels(row, col, s)
+        for delta in (1, 37, 256, 5000):
+            for lv, lo, hi in SCHEDS[:4]:
+                d, st = ctx.sssp(G, s, _sched(npa, lv, lo, hi, drive, 0, delta=delta))
+                assert np.array_equal(d, ref), (drive, delta, lv, lo, hi)
+            lev, _ = ctx.bfs(G, s, _sched(npa, 2, 6, 6, drive, 0, delta=delta))
+            assert np.array_equal(lev, lev_ref), (drive, delta)
+    # delta on a high-diameter graph with zero-weight and unit edges
+    rng = np.random.default_rng(5)
+    row, col, w = random_csr(rng, 5000, 12000, 0, 2)
+    G2 = npa.Graph.from_csr(ctx, row, col, w)
+    ref, _ = orc.dijkstra(row, col, w, 0)
+    for delta in (1, 2, 3):
+        d, _ = ctx.sssp(G2, 0, _sched(npa, 1, 6, 6, drive, 0, delta=delta))
+        assert np.array_equal(d, ref), delta
+    G2.free()
+    G.free()
+
+
 def test_er_64k_config(npa, ctx, orc):
     """Directed ER n=65,536, m=1,048,576, weights U[1,100], source 0, block level."""
     G = npa.Graph.er(ctx, 65536, 1048576, 1, 100, seed=1)
