@@ -1,19 +1,24 @@
 """Benchmark of the SSSP nested-parallelism path on one MI355X.
 
-Workload: SSSP on R-MAT scale 24 (16,777,216 vertices, 268,435,456 directed edges,
-a=.57 b=.19 c=.19, vertex ids permuted, duplicates kept), int32 weights U[1,255],
-16 sources drawn with seed 3 among vertices of out-degree >= 1.  A step solves all 16
-sources.  Modes (same graph, same sources, distances verified identical):
+Default workload (the one the headline metric is quoted on, --config c3): SSSP on R-MAT
+scale 24 (16,777,216 vertices, 268,435,456 directed edges, a=.57 b=.19 c=.19, vertex ids
+permuted, duplicates kept), int32 weights U[1,255], 16 sources drawn with seed 3 among
+vertices of out-degree >= 1.  A step solves every instance once.  Modes (same inputs,
+distances verified identical):
   plain    the reference's fixed schedule: block-level consolidation, THRESHOLD 64,
            one persistent launch per source
   tuned    bins chosen by the autotuner, one launch per source
   learned  the trained MLP picks level and bins per iteration, one launch per source
-  batched  the 16 sources in one persistent launch (learned schedule when a trained
-           model is present, else the tuned one) -- the headline path
-Every run checks all n distances of every source with the shortest-path certificate
-(oracle, O(m)) and one source bit-exactly against the CPU Dijkstra.
+  batched  all sources in one persistent launch -- the headline path
+The other configurations (--config c1|c2|c4|c5) are the parity/evaluation cases of the
+same path (ER 64K, R-MAT s22 per level, many sources + many uneven graphs, grid road
+graph); they print the same JSON schema.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1]
+The CPU leg (skipped with --no-cpu) is the checker and the CPU baseline: every distance of
+every instance and mode is checked with the shortest-path certificate, a bounded sample is
+re-solved with the CPU Dijkstra (bit-exact) and timed.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--config c3]
 Prints one JSON line.
 """
 import argparse
@@ -47,54 +52,155 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+class Workload:
+    """Instances (graph, source) of one configuration."""
+
+    def __init__(self, name, desc, graphs, inst, cpu_sample):
+        self.name, self.desc = name, desc
+        self.graphs = graphs          # distinct npa.Graph objects
+        self.inst = inst              # list of (graph index, source)
+        self.cpu_sample = cpu_sample  # instances re-solved by the CPU Dijkstra
+
+
+def build_workload(npa, ctx, cfg, args):
+    if cfg == "c1":
+        g = npa.Graph.er(ctx, 65536, 1048576, 1, 100, seed=1)
+        return Workload(cfg, "SSSP, directed Erdos-Renyi n=65,536 m=1,048,576 (self-loops redrawn), weights "
+                             "int32 U[1,100], source 0", [g], [(0, 0)], 1)
+    if cfg in ("c2", "c3"):
+        scale, k, seed = (22, 8, 2) if cfg == "c2" else (args.scale, args.sources, 3)
+        g = npa.Graph.rmat(ctx, scale, 16, 0.57, 0.19, 0.19, 1, 255, seed=seed, permute=True)
+        row = g.download()[0]
+        srcs = npa.pick_sources(row, k, seed)
+        return Workload(cfg, "SSSP, R-MAT scale %d, edge factor 16, a=.57 b=.19 c=.19, ids permuted, weights int32 "
+                             "U[1,255], %d sources (seed %d)" % (scale, k, seed), [g],
+                        [(0, int(s)) for s in srcs], 1)
+    if cfg == "c4":
+        big = npa.Graph.rmat(ctx, 20, 16, 0.57, 0.19, 0.19, 1, 255, seed=4, permute=True)
+        row = big.download()[0]
+        srcs = npa.pick_sources(row, 64, 4)
+        graphs, inst = [big], [(0, int(s)) for s in srcs]
+        for i in range(256):
+            seed = 1000 + i
+            scale = 12 + int(npa_scale_draw(seed) % 7)
+            g = npa.Graph.rmat(ctx, scale, 16, 0.57, 0.19, 0.19, 1, 255, seed=seed, permute=True)
+            s = int(npa.pick_sources(g.download()[0], 1, seed)[0])
+            graphs.append(g)
+            inst.append((len(graphs) - 1, s))
+        return Workload(cfg, "SSSP, 64 sources (seed 4) on R-MAT s20 + 256 R-MAT graphs of scale U{12..18} (seeds "
+                             "1000-1255, one source each), edge factor 16, weights int32 U[1,255]", graphs, inst, 2)
+    if cfg == "c5":
+        g = npa.Graph.grid(ctx, 4096, 4096, None, 1, 1000, seed=5)
+        row = g.download()[0]
+        srcs = npa.pick_sources(row, 32, 5)
+        return Workload(cfg, "SSSP, 4096x4096 grid, 4-neighbour bidirectional + 1%% random long-range edges (seed 5), "
+                             "weights int32 U[1,1000], 32 sources (seed 5)", [g], [(0, int(s)) for s in srcs], 1)
+    raise SystemExit("unknown config %s" % cfg)
+
+
+def npa_scale_draw(seed):
+    """Scale draw of the uneven-graph batch: SplitMix64 finaliser of the graph seed."""
+    z = (seed * 0x9E3779B97F4A7C15 + 7) & 0xFFFFFFFFFFFFFFFF
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+    return z ^ (z >> 31)
+
+
+def time_mode(ctx, step, steps, warmup):
+    for _ in range(warmup):
+        step()
+    ctx.sync()
+    t = time.perf_counter()
+    launch_ms = []
+    for _ in range(steps):
+        launch_ms += step()
+    ctx.sync()
+    return (time.perf_counter() - t) * 1e3 / steps, launch_ms
+
+
+def cpu_leg(W, hosts, finals, modes, m_r, k):
+    """Checker and CPU baseline (the only place the oracle is used)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    exact = True
+    t = time.time()
+    for i, (gi, s) in enumerate(W.inst):
+        row, col, w = hosts[gi]
+        code, bad = oracle.check_sssp(row, col, w, s, finals[modes[0]][i])
+        if code != 0:
+            exact = False
+            log("CERTIFICATE FAILED instance %d source %d: %s at vertex %d" % (i, s, oracle.CHECK_CODES[code], bad))
+        for mode in modes[1:]:
+            if not np.array_equal(finals[mode][i], finals[modes[0]][i]):
+                exact = False
+                log("MODE MISMATCH %s vs %s instance %d" % (mode, modes[0], i))
+    log("certificates: %d instances x %d modes in %.1fs: %s" % (len(W.inst), len(modes), time.time() - t, exact))
+    k = min(k, len(W.inst))
+    t = time.perf_counter()
+    refs = []
+    for i in range(k):
+        gi, s = W.inst[i]
+        row, col, w = hosts[gi]
+        refs.append(oracle.dijkstra(row, col, w, s)[0])
+    tc = time.perf_counter() - t
+    for i in range(k):
+        if not np.array_equal(refs[i], finals[modes[0]][i]):
+            exact = False
+            log("DIJKSTRA MISMATCH instance %d" % i)
+    cpu = {"value": round(float(sum(m_r[:k])) / tc / 1e9, 5), "unit": "GTEPS", "cores": 1, "kind": "port",
+           "sample": "CPU Dijkstra (restated SSSP_CPU + binary heap) on %d of the %d instances of the same workload, "
+                     "one thread, %.1f s; distances bit-exact vs the GPU" % (k, len(W.inst), tc)}
+    log("cpu baseline: %.5f GTEPS (%.1f s for %d instance(s))" % (cpu["value"], tc, k))
+    return cpu, exact
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c3")
     ap.add_argument("--scale", type=int, default=24)
     ap.add_argument("--sources", type=int, default=16)
-    ap.add_argument("--seed", type=int, default=3)
-    ap.add_argument("--cpu-sources", type=int, default=1)
+    ap.add_argument("--cpu-sources", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--modes", default="plain,tuned,learned,batched")
+    ap.add_argument("--modes", default=None)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only; --gpus %d is not supported "
-              "(sources and graphs shard across GPUs with no communication; see DESIGN.md)" % args.gpus,
-              file=sys.stderr)
+              "(instances shard across GPUs with no communication; see DESIGN.md)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     npa = load_pkg()
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle
-
     ctx = npa.Context(0)
     ncu, occ = ctx.device_info()
     log("device: %d CUs, persistent occupancy %d workgroups/CU" % (ncu, occ))
     t0 = time.time()
-    G = npa.Graph.rmat(ctx, args.scale, 16, 0.57, 0.19, 0.19, 1, 255, seed=args.seed, permute=True)
-    row, col, w = G.download()
-    log("graph: R-MAT s%d n=%d m=%d generated in %.2fs" % (args.scale, G.n, G.m, time.time() - t0))
-    srcs = npa.pick_sources(row, args.sources, args.seed)
+    W = build_workload(npa, ctx, args.config, args)
+    log("%s: %d graphs, %d instances, built in %.1fs" % (W.desc, len(W.graphs), len(W.inst), time.time() - t0))
 
-    # reached work per source (n_r, m_r) -> algorithmic bytes 12*m_r + 16*n_r
-    dbuf = npa.DeviceBuffer(ctx, G.n)
     plain = npa.Sched(level=1)
+    nmax = max(g.n for g in W.graphs)
+    dbuf = npa.DeviceBuffer(ctx, nmax)
     work = []
-    for s in srcs:
-        ctx.sssp_device(G, int(s), dbuf.ptr, plain)
-        work.append(ctx.reached_work(G, dbuf.ptr))
+    for gi, s in W.inst:
+        ctx.sssp_device(W.graphs[gi], s, dbuf.ptr, plain)
+        work.append(ctx.reached_work(W.graphs[gi], dbuf.ptr))
     n_r = np.array([a for a, _ in work], np.int64)
     m_r = np.array([b for _, b in work], np.int64)
     total_m = int(m_r.sum())
     b_alg = 12 * m_r + 16 * n_r
 
-    modes = [m for m in args.modes.split(",") if m]
-    scheds = {"plain": plain}
+    default_modes = {"c1": "plain", "c2": "level0,level1,level2,tuned", "c3": "plain,tuned,learned,batched",
+                     "c4": "plain,tuned,batched", "c5": "plain,tuned,learned,batched"}[args.config]
+    modes = [m for m in (args.modes or default_modes).split(",") if m]
+    scheds = {"plain": plain, "level0": npa.Sched(level=0), "level1": npa.Sched(level=1),
+              "level2": npa.Sched(level=2)}
     if "tuned" in modes or "batched" in modes:
         t = time.time()
-        tuned, tms = ctx.autotune(G, srcs[:2], npa.Sched(level=1))
+        big = max(range(len(W.graphs)), key=lambda i: W.graphs[i].m)
+        srcs = [s for gi, s in W.inst if gi == big][:2]
+        tuned, tms = ctx.autotune(W.graphs[big], srcs, npa.Sched(level=1))
         scheds["tuned"] = tuned.copy(trace=0)
         log("autotune: %r (%.3f ms/source, %.1fs)" % (tuned, tms, time.time() - t))
     model_path = os.path.join(PKG_DIR, "sched_mlp.f32")
@@ -107,91 +213,53 @@ def main():
         modes = [m for m in modes if m != "learned"]
     batch_sched = scheds["learned"] if have_model else scheds.get("tuned", plain)
 
-    results = {}
-    finals = {}
+    results, finals = {}, {}
     B = None
     for mode in modes:
         if mode == "batched":
-            B = npa.Batch(ctx, [G] * len(srcs), srcs)
+            B = npa.Batch(ctx, [W.graphs[gi] for gi, _ in W.inst], [s for _, s in W.inst])
 
             def step():
-                st = B.solve(batch_sched)
-                return [st.kernel_ms]
+                return [B.solve(batch_sched).kernel_ms]
         else:
             sch = scheds[mode]
 
             def step(sch=sch):
-                return [ctx.sssp_device(G, int(s), dbuf.ptr, sch).kernel_ms for s in srcs]
-        for _ in range(args.warmup):
-            step()
-        ctx.sync()
-        t = time.perf_counter()
-        launch_ms = []
-        for _ in range(args.steps):
-            launch_ms += step()
-        ctx.sync()
-        elapsed = time.perf_counter() - t
-        ms_step = elapsed * 1e3 / args.steps
+                return [ctx.sssp_device(W.graphs[gi], s, dbuf.ptr, sch).kernel_ms for gi, s in W.inst]
+        ms_step, launch_ms = time_mode(ctx, step, args.steps, args.warmup)
         results[mode] = {"ms_per_step": ms_step, "gteps": total_m / (ms_step * 1e-3) / 1e9,
                          "launch_ms": float(np.mean(launch_ms)), "launches_per_step": len(launch_ms) // args.steps}
-        log("%-8s %8.2f ms/step  %7.2f GTEPS  (%d launches/step, %.3f ms each)" % (
+        log("%-8s %9.2f ms/step %8.3f GTEPS  (%d launches/step, %.3f ms each)" % (
             mode, ms_step, results[mode]["gteps"], results[mode]["launches_per_step"], results[mode]["launch_ms"]))
-        # final distances of this mode, all sources
         if mode == "batched":
-            finals[mode] = [B.get(i) for i in range(len(srcs))]
+            finals[mode] = [B.get(i) for i in range(len(W.inst))]
         else:
             out = []
-            for s in srcs:
-                ctx.sssp_device(G, int(s), dbuf.ptr, scheds[mode])
-                out.append(dbuf.to_host())
+            for gi, s in W.inst:
+                ctx.sssp_device(W.graphs[gi], s, dbuf.ptr, scheds[mode])
+                out.append(dbuf.to_host()[:W.graphs[gi].n])
             finals[mode] = out
 
-    # exactness: certificate for every source, identical across modes
-    t = time.time()
-    ref_mode = modes[0]
-    exact = True
-    for i, s in enumerate(srcs):
-        code, bad = oracle.check_sssp(row, col, w, int(s), finals[ref_mode][i])
-        if code != 0:
-            exact = False
-            log("CERTIFICATE FAILED source %d: %s at vertex %d" % (s, oracle.CHECK_CODES[code], bad))
-        for mode in modes[1:]:
-            if not np.array_equal(finals[mode][i], finals[ref_mode][i]):
-                exact = False
-                log("MODE MISMATCH %s vs %s source %d" % (mode, ref_mode, s))
-    log("certificates for %d sources x %d modes in %.1fs: %s" % (len(srcs), len(modes), time.time() - t, exact))
+    cpu, exact = None, None
+    if not args.no_cpu:
+        hosts = [g.download() for g in W.graphs]
+        k = args.cpu_sources if args.cpu_sources is not None else W.cpu_sample
+        cpu, exact = cpu_leg(W, hosts, finals, modes, m_r, k)
 
-    cpu = None
-    if not args.no_cpu and args.cpu_sources > 0:
-        k = min(args.cpu_sources, len(srcs))
-        t = time.perf_counter()
-        refd = [oracle.dijkstra(row, col, w, int(srcs[i]))[0] for i in range(k)]
-        tc = time.perf_counter() - t
-        for i in range(k):
-            if not np.array_equal(refd[i], finals[ref_mode][i]):
-                exact = False
-                log("DIJKSTRA MISMATCH source %d" % srcs[i])
-        cpu = {"value": round(float(m_r[:k].sum()) / tc / 1e9, 4), "unit": "GTEPS", "cores": 1, "kind": "port",
-               "sample": "CPU Dijkstra (restated SSSP_CPU + binary heap) on %d of the %d sources of the same "
-                         "graph, one thread, %.1f s; distances bit-exact vs the GPU" % (k, len(srcs), tc)}
-        log("cpu baseline: %.4f GTEPS (%.1f s for %d source)" % (cpu["value"], tc, k))
-
-    head = "batched" if "batched" in results else modes[-1]
+    head = "batched" if "batched" in results else max(results, key=lambda m: results[m]["gteps"])
     hr = results[head]
-    # roofline of the dominant kernel (the persistent solve): algorithmic bytes of one
-    # launch over its average duration (hipEvents on the launch stream)
     per_launch_bytes = float(b_alg.sum()) if head == "batched" else float(b_alg.mean())
     achieved = per_launch_bytes / (hr["launch_ms"] * 1e-3) / 1e9
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_rmat_s24.json")
+    tpath = os.path.join(ROOT, "profiles", "traffic_%s.json" % args.config)
     if os.path.exists(tpath):
         try:
             traffic = json.load(open(tpath)).get(head)
-        except Exception:
+        except (ValueError, OSError):
             traffic = None
-    plain_g = results.get("plain", {}).get("gteps")
+    base = results.get("plain", {}).get("gteps")
     line = {
-        "metric": METRIC,
+        "metric": METRIC if args.config == "c3" else METRIC + " [config %s]" % args.config,
         "value": round(hr["gteps"], 3),
         "unit": "GTEPS",
         "n_gpus": 1,
@@ -203,30 +271,29 @@ def main():
         "vs_baseline": None,
         "dtype": "int32",
         "data": "synthetic",
-        "config": {"workload": "SSSP, R-MAT scale %d, edge factor 16, a=.57 b=.19 c=.19, ids permuted, "
-                               "weights int32 U[1,255], %d sources (seed %d), one step = all sources"
-                               % (args.scale, len(srcs), args.seed),
-                   "path": head, "schedule": repr(batch_sched if head == "batched" else scheds.get(head)),
-                   "n": G.n, "m": G.m, "reached_edges_per_step": total_m},
+        "config": {"workload": W.desc + "; one step solves every instance", "path": head,
+                   "schedule": repr(batch_sched if head == "batched" else scheds.get(head)),
+                   "instances": len(W.inst), "graphs": len(W.graphs),
+                   "n": int(sum(g.n for g in W.graphs)), "m": int(sum(g.m for g in W.graphs)),
+                   "reached_edges_per_step": total_m},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "k_persistent", "bytes_per_launch": per_launch_bytes},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": "k_persistent",
+                     "bytes_per_launch": per_launch_bytes},
         "cpu_baseline": cpu,
-        "exact": bool(exact),
-        "modes": {k: {"gteps": round(v["gteps"], 3), "ms_per_step": round(v["ms_per_step"], 3)}
-                  for k, v in results.items()},
-        "speedup_learned_vs_plain": round(results["learned"]["gteps"] / plain_g, 3)
-        if plain_g and "learned" in results else None,
-        "speedup_batched_vs_plain": round(results["batched"]["gteps"] / plain_g, 3)
-        if plain_g and "batched" in results else None,
+        "exact": exact,
+        "modes": {k: {"gteps": round(v["gteps"], 3), "ms_per_step": round(v["ms_per_step"], 3),
+                      "launch_ms": round(v["launch_ms"], 4)} for k, v in results.items()},
+        "speedup_learned_vs_plain": round(results["learned"]["gteps"] / base, 3) if base and "learned" in results else None,
+        "speedup_batched_vs_plain": round(results["batched"]["gteps"] / base, 3) if base and "batched" in results else None,
     }
     print(json.dumps(line), flush=True)
     if B is not None:
         B.free()
     dbuf.free()
-    G.free()
+    for g in W.graphs:
+        g.free()
     ctx.close()
-    if not exact:
+    if exact is False:
         sys.exit(1)
 
 

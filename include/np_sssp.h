@@ -128,6 +128,18 @@ int np_destroy(np_ctx *ctx);
 /* Number of compute units and the persistent-kernel occupancy (diagnostics). */
 int np_device_info(np_ctx *ctx, int32_t *num_cus, int32_t *max_wgs_per_cu);
 
+/* Context options (no counterpart in the reference).
+ * NP_CTX_RELABEL: solver vertex numbering of graphs created afterwards.
+ *   0: the caller's ids; 1: always renumber by decreasing in-degree; 2 (default): renumber
+ *   when the n/16 most-targeted vertices receive at least 40% of the edges.  Renumbering
+ *   packs the distances that most relaxations touch into few cache lines; sources and
+ *   results stay in the caller's ids (the mapping is applied on the device). */
+#define NP_CTX_RELABEL 1
+int np_set_option(np_ctx *ctx, int32_t key, int32_t value);
+/* 1 when the graph is solved in a renumbered layout, else 0; skew (optional) receives the
+ * share of edges that its n/16 most-targeted vertices receive. */
+int np_graph_relabeled(const np_graph *g, double *skew);
+
 /* Device-memory plumbing for callers without their own HIP runtime (no counterpart in
  * the reference; its wrapper calls cudaMalloc/cudaMemcpy directly). */
 int np_dev_alloc(np_ctx *ctx, uint64_t bytes, void **d_ptr);

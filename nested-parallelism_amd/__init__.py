@@ -25,6 +25,7 @@ INF = 1073741824          # common/global.h:19
 UNDEFINED = 0x7FFFFFFF    # apps/bfs-rec/src/gpu/bfs_rec.h:13
 NCLASS = 32
 MLP_NIN = 10
+CTX_RELABEL = 1           # np_set_option key: 0 caller ids, 1 always renumber, 2 when skewed
 MLP_NPARAM = 32 * 10 + 32 + 32 * 32 + 32 + 5 * 32 + 5
 
 LEVEL_WAVE, LEVEL_WORKGROUP, LEVEL_GRID, LEVEL_LEARNED = 0, 1, 2, -1
@@ -92,6 +93,8 @@ _SIGS = [
     ("np_get_stream", _vp, [_vp]),
     ("np_destroy", ctypes.c_int, [_vp]),
     ("np_device_info", ctypes.c_int, [_vp, _i32p, _i32p]),
+    ("np_set_option", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32]),
+    ("np_graph_relabeled", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_double)]),
     ("np_dev_alloc", ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.POINTER(_vp)]),
     ("np_dev_free", ctypes.c_int, [_vp, _vp]),
     ("np_dev_copy", ctypes.c_int, [_vp, _vp, _vp, ctypes.c_uint64, ctypes.c_int32]),
@@ -302,6 +305,9 @@ class Context:
     def stream(self):
         return load().np_get_stream(self.h)
 
+    def set_option(self, key, value):
+        _check(load().np_set_option(self.h, int(key), int(value)))
+
     def device_info(self):
         a, b = ctypes.c_int32(), ctypes.c_int32()
         _check(load().np_device_info(self.h, ctypes.byref(a), ctypes.byref(b)))
@@ -452,6 +458,15 @@ class Graph:
         n, m = ctypes.c_int32(), ctypes.c_int64()
         _check(load().np_graph_info(handle, ctypes.byref(n), ctypes.byref(m)))
         self.n, self.m = n.value, m.value
+
+    @property
+    def relabeled(self):
+        """(renumbered?, in-degree skew) of the solver layout."""
+        sk = ctypes.c_double()
+        r = load().np_graph_relabeled(self.h, ctypes.byref(sk))
+        if r < 0:
+            _check(r)
+        return bool(r), sk.value
 
     @classmethod
     def from_csr(cls, ctx, row, col, w):

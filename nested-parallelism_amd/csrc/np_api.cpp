@@ -117,18 +117,32 @@ struct np_ctx {
     int *snap_dist = nullptr;
     unsigned *snap_bm = nullptr;
     long long snap_n = 0;
+    int relabel = 2;  // NP_CTX_RELABEL: 0 never, 1 always, 2 when in-degree is skewed
 };
 
 struct np_graph {
     np_ctx *ctx = nullptr;
     int n = 0;
     long long m = 0;
-    int *row = nullptr, *col = nullptr, *w = nullptr;
+    int *row = nullptr, *col = nullptr, *w = nullptr;  // the caller's CSR (original ids)
     bool owns = false;
-    unsigned char *cls = nullptr;
+    Relabel rl;                                         // solver layout, when renumbered
+    const int *srow = nullptr, *scol = nullptr, *sw = nullptr;  // CSR the kernels run on
+    unsigned char *cls = nullptr;                        // degree class per solver id
     int hist[NCLS] = {0};
     Space sp;  // single-source solver state (lazily allocated)
     bool sp_ready = false;
+    int *out = nullptr;  // un-permuted distances (renumbered graphs)
+
+    // solver id of original vertex v (one 4-byte read when renumbered)
+    int solver_id(int v, hipStream_t s) const {
+        if (!rl.active) return v;
+        int r = -1;
+        if (hipMemcpyAsync(&r, rl.old2new + v, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            return -1;
+        return r;
+    }
 };
 
 struct np_batch {
@@ -136,9 +150,11 @@ struct np_batch {
     int count = 0;
     int unit = 0;
     std::vector<int> base, nverts;
+    std::vector<np_graph *> graphs;                    // graph of each instance
     int *row = nullptr, *col = nullptr, *w = nullptr;  // concatenated graphs (or borrowed)
     bool owns_graph = false;
     Space sp;
+    int *out = nullptr;                                // distances mapped back to original ids
 };
 
 // ------------------------------------------------------------------------------------
@@ -356,39 +372,62 @@ static int solve_single(np_ctx *c, np_graph *g, int src, const np_sched *sched, 
     if (rc) return rc;
     if ((rc = graph_space(c, g))) return rc;
     Space &s = g->sp;
-    int *dist = d_dist_user ? d_dist_user : s.dist;
+    const int ssrc = g->solver_id(src, c->stream);
+    if (ssrc < 0) { set_error("source mapping failed"); return NP_ERR_HIP; }
+    // a renumbered graph is solved in its own buffer, then mapped back to original ids
+    int *dist = (d_dist_user && !g->rl.active) ? d_dist_user : s.dist;
     const int inf = unit ? UNDEFINED : INF;
-    DevState st = make_state(c, s, g->row, g->col, unit ? nullptr : g->w, dist, inf);
+    DevState st = make_state(c, s, g->srow, g->scol, unit ? nullptr : g->sw, dist, inf);
     st.cached_filter = (sch.options & NP_OPT_CACHED_FILTER) ? 1 : 0;
     SchedDev sd = make_schedd(c, sch, g->n);
-    HIPCHK(hipMemcpyAsync(s.src_slots, &src, sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(s.src_slots, &ssrc, sizeof(int), hipMemcpyHostToDevice, c->stream));
     HIPCHK(launch_init_solve(st, s.src_slots, 1, sd.push, sd.delta, c->stream));
     rc = run_drive(c, s, st, sch, sd, false, unit, stats);
     if (rc) return rc;
-    if (h_dist) {
-        HIPCHK(hipMemcpyAsync(h_dist, dist, sizeof(int) * (size_t)g->n, hipMemcpyDeviceToHost, c->stream));
-        if ((rc = ctx_sync(c))) return rc;
+    const int *result = dist;
+    if (g->rl.active) {
+        int *target = d_dist_user;
+        if (!target) {
+            if (!g->out) HIPCHK(dalloc(&g->out, (size_t)g->n));
+            target = g->out;
+        }
+        HIPCHK(launch_unpermute(dist, g->rl.old2new, g->n, target, c->stream));
+        result = target;
     }
-    return NP_OK;
+    if (h_dist) {
+        HIPCHK(hipMemcpyAsync(h_dist, result, sizeof(int) * (size_t)g->n, hipMemcpyDeviceToHost, c->stream));
+    }
+    return ctx_sync(c);
 }
 
 static int finish_graph(np_ctx *c, np_graph *g) {
     int *hist = nullptr;
     int bad = 0;
-    HIPCHK(dalloc(&g->cls, (size_t)g->n));
     HIPCHK(dalloc(&hist, NCLS + 1));
     hipError_t e = launch_fill_i32(hist, NCLS + 1, 0, c->stream);
     if (e == hipSuccess) e = launch_validate_csr(g->row, g->col, g->n, g->m, hist + NCLS, c->stream);
-    if (e == hipSuccess) e = launch_degree_class(g->row, g->n, g->cls, hist, c->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(g->hist, hist, sizeof(g->hist), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(&bad, hist + NCLS, sizeof(int), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    dfree(hist);
-    if (e != hipSuccess) return hip_fail(e, "degree classes", __LINE__);
+    if (e != hipSuccess) { dfree(hist); return hip_fail(e, "CSR validation", __LINE__); }
     if (bad) {
+        dfree(hist);
         set_error("invalid CSR: %d violations of row[0]=0, row non-decreasing, row[n]=m, 0<=col<n", bad);
         return NP_ERR_ARG;
     }
+    g->srow = g->row; g->scol = g->col; g->sw = g->w;
+    if (c->relabel != 0) {
+        const double min_skew = c->relabel == 1 ? -1.0 : 0.4;
+        e = relabel_by_indegree(g->row, g->col, g->w, g->n, g->m, min_skew, &g->rl, c->stream);
+        if (e != hipSuccess) { dfree(hist); return hip_fail(e, "in-degree renumbering", __LINE__); }
+        if (g->rl.active) { g->srow = g->rl.row; g->scol = g->rl.col; g->sw = g->rl.w; }
+    }
+    e = dalloc(&g->cls, (size_t)g->n);
+    if (e == hipSuccess) e = launch_fill_i32(hist, NCLS, 0, c->stream);
+    if (e == hipSuccess) e = launch_degree_class(g->srow, g->n, g->cls, hist, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(g->hist, hist, sizeof(g->hist), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    dfree(hist);
+    if (e != hipSuccess) return hip_fail(e, "degree classes", __LINE__);
     return NP_OK;
 }
 
@@ -466,6 +505,25 @@ int np_set_stream(np_ctx *c, void *stream) {
 }
 
 void *np_get_stream(np_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+int np_set_option(np_ctx *c, int32_t key, int32_t value) {
+    if (!c) { set_error("null context"); return NP_ERR_ARG; }
+    switch (key) {
+    case NP_CTX_RELABEL:
+        if (value < 0 || value > 2) { set_error("NP_CTX_RELABEL takes 0, 1 or 2"); return NP_ERR_ARG; }
+        c->relabel = value;
+        return NP_OK;
+    default:
+        set_error("unknown option");
+        return NP_ERR_ARG;
+    }
+}
+
+int np_graph_relabeled(const np_graph *g, double *skew) {
+    if (!g) { set_error("null graph"); return NP_ERR_ARG; }
+    if (skew) *skew = g->rl.skew;
+    return g->rl.active ? 1 : 0;
+}
 
 int np_destroy(np_ctx *c) {
     if (!c) return NP_OK;
@@ -591,7 +649,9 @@ int np_graph_free(np_graph *g) {
     if (g->ctx && g->ctx->step_g == g) g->ctx->step_g = nullptr;
     g->sp.release();
     if (g->owns) { dfree(g->row); dfree(g->col); dfree(g->w); }
+    dfree(g->rl.row); dfree(g->rl.col); dfree(g->rl.w); dfree(g->rl.old2new);
     dfree(g->cls);
+    dfree(g->out);
     delete g;
     return NP_OK;
 }
@@ -657,8 +717,10 @@ int np_batch_create(np_ctx *c, int32_t count, np_graph *const *graphs, const int
     Space &s = b->sp;
     std::vector<long long> VB(uniq.size()), EB(uniq.size());
     int rc = NP_OK;
+    // every graph runs in its solver layout (renumbered when skewed)
     if (uniq.size() == 1) {
-        b->row = uniq[0]->row; b->col = uniq[0]->col; b->w = uniq[0]->w;
+        b->row = const_cast<int *>(uniq[0]->srow); b->col = const_cast<int *>(uniq[0]->scol);
+        b->w = const_cast<int *>(uniq[0]->sw);
         b->owns_graph = false;
         VB[0] = 0; EB[0] = 0;
     } else {
@@ -671,22 +733,22 @@ int np_batch_create(np_ctx *c, int32_t count, np_graph *const *graphs, const int
         if (e == hipSuccess) e = dalloc(&b->w, (size_t)ne);
         for (size_t j = 0; j < uniq.size() && e == hipSuccess; ++j) {
             np_graph *g = uniq[j];
-            e = launch_rebase_row(g->row, g->n, (int)EB[j], b->row + VB[j], c->stream);
+            e = launch_rebase_row(g->srow, g->n, (int)EB[j], b->row + VB[j], c->stream);
             if (e == hipSuccess && g->m)
-                e = hipMemcpyAsync(b->col + EB[j], g->col, sizeof(int) * (size_t)g->m, hipMemcpyDeviceToDevice, c->stream);
+                e = hipMemcpyAsync(b->col + EB[j], g->scol, sizeof(int) * (size_t)g->m, hipMemcpyDeviceToDevice, c->stream);
             if (e == hipSuccess && g->m)
-                e = hipMemcpyAsync(b->w + EB[j], g->w, sizeof(int) * (size_t)g->m, hipMemcpyDeviceToDevice, c->stream);
+                e = hipMemcpyAsync(b->w + EB[j], g->sw, sizeof(int) * (size_t)g->m, hipMemcpyDeviceToDevice, c->stream);
         }
         if (e != hipSuccess) { rc = hip_fail(e, "batch graph concat", __LINE__); np_batch_free(b); return rc; }
     }
     // slots: instance i owns [base_i, base_i + pad(n_i)), pad to 2^LUT_SHIFT
     const long long G = 1ll << LUT_SHIFT;
     long long acc = 0;
-    b->base.resize(count); b->nverts.resize(count);
+    b->base.resize(count); b->nverts.resize(count); b->graphs.resize(count);
     std::vector<int2> inst(count);
     for (int i = 0; i < count; ++i) {
         np_graph *g = uniq[gi[i]];
-        b->base[i] = (int)acc; b->nverts[i] = g->n;
+        b->base[i] = (int)acc; b->nverts[i] = g->n; b->graphs[i] = g;
         inst[i] = make_int2((int)acc, (int)VB[gi[i]]);
         for (int k = 0; k < NCLS; ++k) s.hist[k] += g->hist[k];
         acc += ((long long)g->n + G - 1) / G * G;
@@ -708,7 +770,11 @@ int np_batch_create(np_ctx *c, int32_t count, np_graph *const *graphs, const int
     if (e != hipSuccess) { rc = hip_fail(e, "batch setup", __LINE__); np_batch_free(b); return rc; }
     if ((rc = ensure_src(c, s, count))) { np_batch_free(b); return rc; }
     std::vector<int> slots(count);
-    for (int i = 0; i < count; ++i) slots[i] = b->base[i] + srcs[i];
+    for (int i = 0; i < count; ++i) {
+        const int sid = b->graphs[i]->solver_id(srcs[i], c->stream);
+        if (sid < 0) { np_batch_free(b); set_error("source mapping failed"); return NP_ERR_HIP; }
+        slots[i] = b->base[i] + sid;
+    }
     e = hipMemcpyAsync(s.src_slots, slots.data(), sizeof(int) * (size_t)count, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) { rc = hip_fail(e, "batch sources", __LINE__); np_batch_free(b); return rc; }
@@ -732,17 +798,33 @@ int np_batch_solve(np_ctx *c, np_batch *b, const np_sched *sched, np_stats *stat
     return run_drive(c, s, st, sch, sd, true, unit, stats);
 }
 
+// Instance i's distances in original vertex ids (mapped back when its graph is renumbered).
+static int batch_result(np_ctx *c, np_batch *b, int i, const int **res) {
+    const np_graph *g = b->graphs[i];
+    const int *d = b->sp.dist + b->base[i];
+    if (g->rl.active) {
+        if (!b->out) HIPCHK(dalloc(&b->out, (size_t)b->sp.nslots));
+        HIPCHK(launch_unpermute(d, g->rl.old2new, g->n, b->out + b->base[i], c->stream));
+        d = b->out + b->base[i];
+    }
+    *res = d;
+    return NP_OK;
+}
+
 int np_batch_get(np_ctx *c, np_batch *b, int32_t i, int32_t *dist_out) {
     if (!c || !b || !dist_out || i < 0 || i >= b->count) { set_error("bad argument"); return NP_ERR_ARG; }
-    HIPCHK(hipMemcpyAsync(dist_out, b->sp.dist + b->base[i], sizeof(int) * (size_t)b->nverts[i],
-                          hipMemcpyDeviceToHost, c->stream));
+    const int *d = nullptr;
+    int rc = batch_result(c, b, i, &d);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(dist_out, d, sizeof(int) * (size_t)b->nverts[i], hipMemcpyDeviceToHost, c->stream));
     return ctx_sync(c);
 }
 
 int np_batch_device_dist(np_batch *b, int32_t i, const int32_t **d_dist) {
     if (!b || !d_dist || i < 0 || i >= b->count) { set_error("bad argument"); return NP_ERR_ARG; }
-    *d_dist = b->sp.dist + b->base[i];
-    return NP_OK;
+    int rc = batch_result(b->ctx, b, i, d_dist);
+    if (rc) return rc;
+    return ctx_sync(b->ctx);
 }
 
 int np_batch_free(np_batch *b) {
@@ -750,6 +832,7 @@ int np_batch_free(np_batch *b) {
     if (b->ctx) (void)hipSetDevice(b->ctx->device);
     b->sp.release();
     if (b->owns_graph) { dfree(b->row); dfree(b->col); dfree(b->w); }
+    dfree(b->out);
     delete b;
     return NP_OK;
 }
@@ -867,7 +950,7 @@ int np_model_forward(np_ctx *c, const float *x, int32_t batch, float *out) {
 static int step_state(np_ctx *c, DevState &st, SchedDev &sd) {
     np_graph *g = c->step_g;
     if (!g) { set_error("no stepped solve in progress"); return NP_ERR_ARG; }
-    st = make_state(c, g->sp, g->row, g->col, g->w, g->sp.dist, INF);
+    st = make_state(c, g->sp, g->srow, g->scol, g->sw, g->sp.dist, INF);
     np_sched d;
     np_sched_default(1, &d);
     sd = make_schedd(c, d, g->n);
@@ -893,11 +976,13 @@ int np_step_begin(np_ctx *c, np_graph *g, int32_t src, np_stats *stats) {
         HIPCHK(dalloc(&c->snap_bm, (size_t)(g->n + 31) / 32));
         c->snap_n = g->n;
     }
+    const int sid = g->solver_id(src, c->stream);
+    if (sid < 0) { set_error("source mapping failed"); return NP_ERR_HIP; }
     c->step_g = g;
     c->step_it = 0;
     DevState st; SchedDev sd;
     if ((rc = step_state(c, st, sd))) return rc;
-    HIPCHK(hipMemcpyAsync(g->sp.src_slots, &src, sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(g->sp.src_slots, &sid, sizeof(int), hipMemcpyHostToDevice, c->stream));
     HIPCHK(launch_init_solve(st, g->sp.src_slots, 1, 0, 0, c->stream));
     HIPCHK(launch_set_cfg(st, 1, 6, 6, c->stream));
     if (stats) std::memset(stats, 0, sizeof(*stats));
@@ -966,7 +1051,13 @@ int np_step_end(np_ctx *c, int32_t *dist_out) {
     if (!c || !c->step_g) { set_error("no stepped solve in progress"); return NP_ERR_ARG; }
     np_graph *g = c->step_g;
     if (dist_out) {
-        HIPCHK(hipMemcpyAsync(dist_out, g->sp.dist, sizeof(int) * (size_t)g->n, hipMemcpyDeviceToHost, c->stream));
+        const int *d = g->sp.dist;
+        if (g->rl.active) {
+            if (!g->out) HIPCHK(dalloc(&g->out, (size_t)g->n));
+            HIPCHK(launch_unpermute(g->sp.dist, g->rl.old2new, g->n, g->out, c->stream));
+            d = g->out;
+        }
+        HIPCHK(hipMemcpyAsync(dist_out, d, sizeof(int) * (size_t)g->n, hipMemcpyDeviceToHost, c->stream));
         int rc = ctx_sync(c);
         if (rc) return rc;
     }

@@ -7,6 +7,7 @@
 // order -- what convertCSR (common/global.cpp:216-221) makes of a file read in that order.
 #include <hip/hip_runtime.h>
 #include <cstring>
+#include <vector>
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 #include "np_internal.h"
@@ -200,6 +201,120 @@ hipError_t gen_grid_edges(int rows, int cols, long long n_long, uint64_t seed, i
                        cols, grid_m, n_long, grid_off, stream_key(seed, ST_LONG),
                        stream_key(seed, ST_WEIGHT), wmin, (uint32_t)(wmax - wmin + 1), src, dst, w);
     return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------
+// Solver layout: vertices renumbered by decreasing in-degree (ties keep their order), so
+// the targets most relaxations hit -- R-MAT concentrates most in-edges on few vertices --
+// share a few megabytes of the distance array and stay in L2.  The renumbering is
+// internal: sources are mapped in, distances mapped back out.
+// ---------------------------------------------------------------------------------
+__global__ void k_count_indeg(const int *col, long long m, unsigned *indeg) {
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < m;
+         e += (long long)gridDim.x * blockDim.x)
+        atomicAdd(&indeg[col[e]], 1u);
+}
+__global__ void k_indeg_keys(const unsigned *indeg, int n, unsigned *keys, unsigned *vals) {
+    for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += (long long)gridDim.x * blockDim.x) {
+        keys[v] = ~indeg[v];
+        vals[v] = (unsigned)v;
+    }
+}
+__global__ void k_inverse_perm(const unsigned *new2old, int n, int *old2new, int *deg2, const int *row) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        unsigned o = new2old[i];
+        old2new[o] = (int)i;
+        deg2[i] = row[o + 1] - row[o];
+    }
+}
+// One 64-lane wavefront per renumbered vertex copies its edge list.
+__global__ void k_relabel_edges(const int *row, const int *col, const int *w, const unsigned *new2old,
+                                const int *old2new, const int *row2, int n, int *col2, int *w2) {
+    const long long nw = (long long)gridDim.x * (blockDim.x / 64);
+    const int lane = threadIdx.x & 63;
+    for (long long i = (long long)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6); i < n; i += nw) {
+        const unsigned o = new2old[i];
+        const int s = row[o], d = row[o + 1] - s, t = row2[i];
+        for (int k = lane; k < d; k += 64) {
+            col2[t + k] = old2new[col[s + k]];
+            if (w) w2[t + k] = w[s + k];
+        }
+    }
+}
+__global__ void k_unpermute(const int *src, const int *old2new, int n, int *out) {
+    for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += (long long)gridDim.x * blockDim.x)
+        out[v] = src[old2new[v]];
+}
+
+hipError_t launch_unpermute(const int *src, const int *old2new, int n, int *out, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_unpermute, dim3(grid_for(n, 256)), dim3(256), 0, s, src, old2new, n, out);
+    return hipGetLastError();
+}
+
+hipError_t relabel_by_indegree(const int *row, const int *col, const int *w, int n, long long m,
+                               double min_skew, Relabel *out, hipStream_t s) {
+    hipError_t err = hipSuccess;
+    unsigned *indeg = nullptr, *keys = nullptr, *vals = nullptr, *keys2 = nullptr, *vals2 = nullptr;
+    int *deg2 = nullptr;
+    void *tmp = nullptr;
+    size_t tb = 0;
+    *out = Relabel{};
+    if (n < 2 || m < 1) return hipSuccess;
+    NP_CHECK(hipMalloc(&indeg, sizeof(unsigned) * (size_t)n));
+    NP_CHECK(hipMalloc(&keys, sizeof(unsigned) * (size_t)n));
+    NP_CHECK(hipMalloc(&vals, sizeof(unsigned) * (size_t)n));
+    NP_CHECK(hipMalloc(&keys2, sizeof(unsigned) * (size_t)n));
+    NP_CHECK(hipMalloc(&vals2, sizeof(unsigned) * (size_t)n));
+    hipLaunchKernelGGL(k_zero_i32, dim3(grid_for(n, 256)), dim3(256), 0, s, (int *)indeg, (long long)n);
+    hipLaunchKernelGGL(k_count_indeg, dim3(grid_for(m, 256)), dim3(256), 0, s, col, m, indeg);
+    hipLaunchKernelGGL(k_indeg_keys, dim3(grid_for(n, 256)), dim3(256), 0, s, indeg, n, keys, vals);
+    NP_CHECK(rocprim::radix_sort_pairs(nullptr, tb, keys, keys2, vals, vals2, (size_t)n, 0, 32, s));
+    NP_CHECK(hipMalloc(&tmp, tb ? tb : 1));
+    NP_CHECK(rocprim::radix_sort_pairs(tmp, tb, keys, keys2, vals, vals2, (size_t)n, 0, 32, s));
+    {
+        // skew: share of the edges received by the top n/16 vertices
+        const int top = n / 16 > 0 ? n / 16 : 1;
+        std::vector<unsigned> h((size_t)top);
+        NP_CHECK(hipMemcpyAsync(h.data(), keys2, sizeof(unsigned) * (size_t)top, hipMemcpyDeviceToHost, s));
+        NP_CHECK(hipStreamSynchronize(s));
+        double sum = 0;
+        for (unsigned k : h) sum += (double)(~k);
+        out->skew = sum / (double)m;
+    }
+    if (out->skew < min_skew) goto done;
+    NP_CHECK(hipMalloc(&out->row, sizeof(int) * ((size_t)n + 1)));
+    NP_CHECK(hipMalloc(&out->col, sizeof(int) * (size_t)m));
+    NP_CHECK(hipMalloc(&out->w, sizeof(int) * (size_t)m));
+    NP_CHECK(hipMalloc(&out->old2new, sizeof(int) * (size_t)n));
+    NP_CHECK(hipMalloc(&deg2, sizeof(int) * ((size_t)n + 1)));
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, deg2 + n, 1ll);
+    hipLaunchKernelGGL(k_inverse_perm, dim3(grid_for(n, 256)), dim3(256), 0, s, vals2, n, out->old2new, deg2, row);
+    NP_CHECK(hipFree(tmp)); tmp = nullptr;
+    {
+        size_t sb = 0;
+        NP_CHECK(rocprim::exclusive_scan(nullptr, sb, deg2, out->row, 0, (size_t)n + 1, rocprim::plus<int>(), s));
+        NP_CHECK(hipMalloc(&tmp, sb ? sb : 1));
+        NP_CHECK(rocprim::exclusive_scan(tmp, sb, deg2, out->row, 0, (size_t)n + 1, rocprim::plus<int>(), s));
+    }
+    hipLaunchKernelGGL(k_relabel_edges, dim3(grid_for((long long)n * 64, 256) > 16384 ? 16384 : grid_for((long long)n * 64, 256)),
+                       dim3(256), 0, s, row, col, w, vals2, out->old2new, out->row, n, out->col, out->w);
+    NP_CHECK(hipGetLastError());
+    NP_CHECK(hipStreamSynchronize(s));
+    out->active = true;
+done:
+    (void)hipFree(tmp); (void)hipFree(indeg); (void)hipFree(keys); (void)hipFree(vals);
+    (void)hipFree(keys2); (void)hipFree(vals2); (void)hipFree(deg2);
+    if (err != hipSuccess || !out->active) {
+        (void)hipFree(out->row); (void)hipFree(out->col); (void)hipFree(out->w); (void)hipFree(out->old2new);
+        double sk = out->skew;
+        *out = Relabel{};
+        out->skew = sk;
+    }
+    return err;
 }
 
 // Exclusive prefix of grid out-degrees (grid_off[n] = grid edge count).

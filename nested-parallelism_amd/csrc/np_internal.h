@@ -118,6 +118,19 @@ hipError_t gen_grid_edges(int rows, int cols, long long n_long, uint64_t seed, i
                           hipStream_t s);
 
 hipError_t grid_offsets(int rows, int cols, long long *grid_off, hipStream_t s);
+
+// Solver layout renumbered by decreasing in-degree.  `skew` = share of the edges that the
+// top n/16 vertices receive; the layout is built (active) only when skew >= min_skew.
+struct Relabel {
+    bool active = false;
+    double skew = 0.0;
+    int *row = nullptr, *col = nullptr, *w = nullptr;  // renumbered CSR
+    int *old2new = nullptr;                             // original id -> solver id
+};
+hipError_t relabel_by_indegree(const int *row, const int *col, const int *w, int n, long long m,
+                               double min_skew, Relabel *out, hipStream_t s);
+// out[v] = src[old2new[v]]
+hipError_t launch_unpermute(const int *src, const int *old2new, int n, int *out, hipStream_t s);
 // Thresholds of rmat_edge from probabilities.
 void rmat_thresholds(double a, double b, double c, uint64_t *ta, uint64_t *tab, uint64_t *tabc);
 

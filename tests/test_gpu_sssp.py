@@ -1,5 +1,6 @@
 """GPU parity of the SSSP / BFS path: the HIP kernels through the C ABI against the oracle
 and the reference fixtures, bit-exact (int32 distances and levels)."""
+import ctypes
 import glob
 import os
 
@@ -262,3 +263,54 @@ def test_stepping_reaches_exact_distances(npa, ctx, orc):
     ref, _ = orc.dijkstra(row, col, w, s)
     assert np.array_equal(d, ref)
     G.free()
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_relabel_modes(npa, orc, mode):
+    """Solver renumbering by in-degree: sources and results stay in the caller's ids."""
+    c = npa.Context(0)
+    try:
+        c.set_option(npa.CTX_RELABEL, mode)
+        G = npa.Graph.rmat(c, 13, 16, wmin=1, wmax=255, seed=21)
+        rng = np.random.default_rng(21)
+        E = npa.Graph.from_csr(c, *random_csr(rng, 3000, 24000, 1, 50)[:3])
+        on_g, skew = G.relabeled
+        on_e, _ = E.relabeled
+        assert on_g == (mode != 0)
+        if mode:                            # R-MAT: the top n/16 vertices receive most edges
+            assert 0.4 <= skew <= 1.0
+        assert on_e == (mode == 1)
+        row, col, w = G.download()          # always the caller's CSR
+        srcs = [int(x) for x in npa.pick_sources(row, 3, 21)]
+        for s in srcs:
+            ref, _ = orc.dijkstra(row, col, w, s)
+            for lv in (0, 1, 2):
+                d, _ = c.sssp(G, s, _sched(npa, lv, 6, 6))
+                assert np.array_equal(d, ref), (s, lv)
+            d_dist = npa.DeviceBuffer(c, G.n)
+            c.sssp_device(G, s, d_dist.ptr, _sched(npa, 1, 6, 6, 1, 0))
+            assert np.array_equal(d_dist.to_host(), ref)
+            assert c.reached_work(G, d_dist.ptr) == orc.reached_work(row, ref)
+            d_dist.free()
+            lv_ref = orc.bfs_levels(row, col, s)
+            assert np.array_equal(c.bfs(G, s, _sched(npa, 2, 6, 6))[0], lv_ref)
+        er = E.download()
+        B = npa.Batch(c, [G, E, G], [srcs[0], 7, srcs[1]])
+        B.solve(_sched(npa, 1, 6, 6))
+        for i, (gr, s) in enumerate([((row, col, w), srcs[0]), (er, 7), ((row, col, w), srcs[1])]):
+            ref, _ = orc.dijkstra(gr[0], gr[1], gr[2], s)
+            assert np.array_equal(B.get(i), ref), i
+            out = np.empty(len(ref), np.int32)
+            npa._check(npa.load().np_dev_copy(c.h, npa._ptr(out), ctypes.c_void_p(B.device_dist(i)),
+                                              out.nbytes, 1))
+            assert np.array_equal(out, ref), i
+        B.free()
+        c.step_begin(G, srcs[2])
+        while not c.step_frontier()[1]:
+            c.step_advance(npa.Sched(level=1))
+        ref, _ = orc.dijkstra(row, col, w, srcs[2])
+        assert np.array_equal(c.step_end(G.n), ref)
+        G.free()
+        E.free()
+    finally:
+        c.close()
