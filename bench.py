@@ -9,7 +9,10 @@ distances verified identical):
            one persistent launch per source
   tuned    bins chosen by the autotuner, one launch per source
   learned  the trained MLP picks level and bins per iteration, one launch per source
-  batched  all sources in one persistent launch -- the headline path
+  concat   all instances in one persistent launch over a concatenated slot space
+  batched  np_batch with source packing: the sources of one graph share distance rows
+           (one persistent launch per 32 sources), bins from the batch autotuner
+The JSON value is the fastest mode; every mode is listed under "modes".
 The other configurations (--config c1|c2|c4|c5) are the parity/evaluation cases of the
 same path (ER 64K, R-MAT s22 per level, many sources + many uneven graphs, grid road
 graph); they print the same JSON schema.
@@ -182,21 +185,14 @@ def main():
     plain = npa.Sched(level=1)
     nmax = max(g.n for g in W.graphs)
     dbuf = npa.DeviceBuffer(ctx, nmax)
-    work = []
-    for gi, s in W.inst:
-        ctx.sssp_device(W.graphs[gi], s, dbuf.ptr, plain)
-        work.append(ctx.reached_work(W.graphs[gi], dbuf.ptr))
-    n_r = np.array([a for a, _ in work], np.int64)
-    m_r = np.array([b for _, b in work], np.int64)
-    total_m = int(m_r.sum())
-    b_alg = 12 * m_r + 16 * n_r
 
-    default_modes = {"c1": "plain", "c2": "level0,level1,level2,tuned", "c3": "plain,tuned,learned,batched",
-                     "c4": "plain,tuned,batched", "c5": "plain,tuned,learned,batched"}[args.config]
+    default_modes = {"c1": "plain", "c2": "level0,level1,level2,tuned",
+                     "c3": "plain,tuned,learned,concat,batched",
+                     "c4": "plain,tuned,concat,batched", "c5": "plain,tuned,learned,batched"}[args.config]
     modes = [m for m in (args.modes or default_modes).split(",") if m]
     scheds = {"plain": plain, "level0": npa.Sched(level=0), "level1": npa.Sched(level=1),
               "level2": npa.Sched(level=2)}
-    if "tuned" in modes or "batched" in modes:
+    if "tuned" in modes or "concat" in modes:
         t = time.time()
         big = max(range(len(W.graphs)), key=lambda i: W.graphs[i].m)
         srcs = [s for gi, s in W.inst if gi == big][:2]
@@ -204,41 +200,71 @@ def main():
         scheds["tuned"] = tuned.copy(trace=0)
         log("autotune: %r (%.3f ms/source, %.1fs)" % (tuned, tms, time.time() - t))
     model_path = os.path.join(PKG_DIR, "sched_mlp.f32")
-    have_model = os.path.exists(model_path)
-    if have_model:
+    if os.path.exists(model_path):
         ctx.model_set(np.fromfile(model_path, dtype=np.float32))
         scheds["learned"] = npa.Sched(level=npa.LEVEL_LEARNED, log2_t_lo=6, log2_t_hi=6)
     elif "learned" in modes:
         log("no trained scheduler model at %s: learned mode skipped" % model_path)
         modes = [m for m in modes if m != "learned"]
-    batch_sched = scheds["learned"] if have_model else scheds.get("tuned", plain)
 
-    results, finals = {}, {}
-    B = None
+    results, finals, batches = {}, {}, {}
     for mode in modes:
-        if mode == "batched":
+        if mode in ("batched", "concat"):
+            # batched: instances sharing a graph source-packed; concat: every instance in the
+            # concatenated slot space (packing off)
+            ctx.set_option(npa.CTX_PACK, 1 if mode == "batched" else 0)
             B = npa.Batch(ctx, [W.graphs[gi] for gi, _ in W.inst], [s for _, s in W.inst])
+            ctx.set_option(npa.CTX_PACK, 1)
+            batches[mode] = B
+            if mode == "batched":
+                t = time.time()
+                bs, bms = B.autotune(npa.Sched(level=1))
+                scheds[mode] = bs.copy(trace=0)
+                log("batch autotune: %r (%.3f ms/solve, %.1fs); layout %s" % (bs, bms, time.time() - t, B.layout()))
+            else:
+                scheds[mode] = scheds["tuned"]
 
-            def step():
-                return [B.solve(batch_sched).kernel_ms]
+            def step(B=B, sch=scheds[mode]):
+                st = B.solve(sch)
+                if st.drive != npa.DRIVE_PERSISTENT:
+                    log("note: the batch ran on drive %d (no cooperative queue for this process)" % st.drive)
+                return [st.kernel_ms]
         else:
             sch = scheds[mode]
 
             def step(sch=sch):
-                return [ctx.sssp_device(W.graphs[gi], s, dbuf.ptr, sch).kernel_ms for gi, s in W.inst]
+                out = []
+                for gi, s in W.inst:
+                    st = ctx.sssp_device(W.graphs[gi], s, dbuf.ptr, sch)
+                    if st.drive != npa.DRIVE_PERSISTENT:
+                        log("note: a solve ran on drive %d (no cooperative queue for this process)" % st.drive)
+                    out.append(st.kernel_ms)
+                return out
         ms_step, launch_ms = time_mode(ctx, step, args.steps, args.warmup)
-        results[mode] = {"ms_per_step": ms_step, "gteps": total_m / (ms_step * 1e-3) / 1e9,
-                         "launch_ms": float(np.mean(launch_ms)), "launches_per_step": len(launch_ms) // args.steps}
-        log("%-8s %9.2f ms/step %8.3f GTEPS  (%d launches/step, %.3f ms each)" % (
-            mode, ms_step, results[mode]["gteps"], results[mode]["launches_per_step"], results[mode]["launch_ms"]))
-        if mode == "batched":
-            finals[mode] = [B.get(i) for i in range(len(W.inst))]
+        results[mode] = {"ms_per_step": ms_step, "launch_ms": float(np.mean(launch_ms)),
+                         "launches_per_step": len(launch_ms) // args.steps}
+        if mode in batches:
+            finals[mode] = [batches[mode].get(i) for i in range(len(W.inst))]
         else:
             out = []
             for gi, s in W.inst:
                 ctx.sssp_device(W.graphs[gi], s, dbuf.ptr, scheds[mode])
                 out.append(dbuf.to_host()[:W.graphs[gi].n])
             finals[mode] = out
+        if mode == modes[0]:
+            # work counted for throughput, from the distances (identical in every mode)
+            work = []
+            for (gi, s), d in zip(W.inst, finals[mode]):
+                dbuf.from_host(np.concatenate([d, np.zeros(nmax - len(d), np.int32)]))
+                work.append(ctx.reached_work(W.graphs[gi], dbuf.ptr))
+            n_r = np.array([a for a, _ in work], np.int64)
+            m_r = np.array([b for _, b in work], np.int64)
+            total_m = int(m_r.sum())
+            b_alg = 12 * m_r + 16 * n_r
+        results[mode]["gteps"] = total_m / (ms_step * 1e-3) / 1e9
+        log("%-8s %9.2f ms/step %8.3f GTEPS  (%d launches/step, %.3f ms each)  %r" % (
+            mode, ms_step, results[mode]["gteps"], results[mode]["launches_per_step"], results[mode]["launch_ms"],
+            scheds[mode]))
 
     cpu, exact = None, None
     if not args.no_cpu:
@@ -246,9 +272,15 @@ def main():
         k = args.cpu_sources if args.cpu_sources is not None else W.cpu_sample
         cpu, exact = cpu_leg(W, hosts, finals, modes, m_r, k)
 
-    head = "batched" if "batched" in results else max(results, key=lambda m: results[m]["gteps"])
+    head = max(results, key=lambda m: results[m]["gteps"])
     hr = results[head]
-    per_launch_bytes = float(b_alg.sum()) if head == "batched" else float(b_alg.mean())
+    if head in batches:
+        npk, ncat = batches[head].layout()
+        kernel = "k_pk_persistent" if npk and not ncat else ("k_persistent" if not npk else "k_pk_persistent+k_persistent")
+        per_launch_bytes = float(b_alg.sum()) / max(1, hr["launches_per_step"])
+    else:
+        kernel = "k_persistent"
+        per_launch_bytes = float(b_alg.mean())
     achieved = per_launch_bytes / (hr["launch_ms"] * 1e-3) / 1e9
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_%s.json" % args.config)
@@ -272,22 +304,23 @@ def main():
         "dtype": "int32",
         "data": "synthetic",
         "config": {"workload": W.desc + "; one step solves every instance", "path": head,
-                   "schedule": repr(batch_sched if head == "batched" else scheds.get(head)),
+                   "schedule": repr(scheds.get(head)),
                    "instances": len(W.inst), "graphs": len(W.graphs),
                    "n": int(sum(g.n for g in W.graphs)), "m": int(sum(g.m for g in W.graphs)),
                    "reached_edges_per_step": total_m},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": "k_persistent",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kernel,
                      "bytes_per_launch": per_launch_bytes},
         "cpu_baseline": cpu,
         "exact": exact,
         "modes": {k: {"gteps": round(v["gteps"], 3), "ms_per_step": round(v["ms_per_step"], 3),
-                      "launch_ms": round(v["launch_ms"], 4)} for k, v in results.items()},
+                      "launch_ms": round(v["launch_ms"], 4), "schedule": repr(scheds.get(k))}
+                  for k, v in results.items()},
         "speedup_learned_vs_plain": round(results["learned"]["gteps"] / base, 3) if base and "learned" in results else None,
         "speedup_batched_vs_plain": round(results["batched"]["gteps"] / base, 3) if base and "batched" in results else None,
     }
     print(json.dumps(line), flush=True)
-    if B is not None:
+    for B in batches.values():
         B.free()
     dbuf.free()
     for g in W.graphs:

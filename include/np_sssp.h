@@ -109,7 +109,10 @@ typedef struct np_stats {
     int32_t status;         /* same as the return code */
     double kernel_ms;       /* device time of the solve, hipEvents on the context stream */
     int64_t frontier_total; /* sum of frontier sizes over iterations */
-    int64_t reserved[6];
+    int32_t drive;          /* NP_DRIVE_* that ran: a persistent solve runs host-loop when the
+                               process gets no cooperative queue (one holder per device) */
+    int32_t pad;
+    int64_t reserved[5];
 } np_stats;
 
 typedef struct np_ctx np_ctx;
@@ -135,6 +138,9 @@ int np_device_info(np_ctx *ctx, int32_t *num_cus, int32_t *max_wgs_per_cu);
  *   packs the distances that most relaxations touch into few cache lines; sources and
  *   results stay in the caller's ids (the mapping is applied on the device). */
 #define NP_CTX_RELABEL 1
+/* NP_CTX_PACK: 1 (default) batch instances that share a graph are solved source-packed
+ * (np_batch_create); 0 every instance goes to the concatenated part. */
+#define NP_CTX_PACK 2
 int np_set_option(np_ctx *ctx, int32_t key, int32_t value);
 /* 1 when the graph is solved in a renumbered layout, else 0; skew (optional) receives the
  * share of edges that its n/16 most-targeted vertices receive. */
@@ -178,16 +184,27 @@ int np_bfs(np_ctx *ctx, np_graph *g, int32_t src, const np_sched *sched, int32_t
 
 /* Batched solves -- no counterpart in the reference.  `count` instances; instance i
  * solves graph graphs[i] from source srcs[i] (graphs may repeat: many sources on one
- * graph; or differ: many graphs of uneven size).  All instances run in one persistent
- * launch (or one captured graph) over a shared, instance-tagged worklist.  unit != 0
- * solves BFS levels instead of distances. */
+ * graph; or differ: many graphs of uneven size).  unit != 0 solves BFS levels instead of
+ * distances.  Instances are laid out in two kinds of parts, each solved by one launch:
+ *   packs         2..32 sources of a graph that appears more than once share one state
+ *                 whose distances are stored source-minor (the distances of all its
+ *                 sources at a vertex are adjacent), so an edge is read once for every
+ *                 source pending at its tail (NP_CTX_PACK, default on)
+ *   concatenated  every other instance: the graphs laid end to end in one slot space
+ *                 with an instance-tagged worklist.
+ * Packs always use the scan frontier without a priority window (np_sched.frontier and
+ * .delta apply to the concatenated part); every other field applies to both. */
 int np_batch_create(np_ctx *ctx, int32_t count, np_graph *const *graphs, const int32_t *srcs,
                     int32_t unit, np_batch **out);
+/* stats: iterations, kernel_ms and frontier_total summed over the parts. */
 int np_batch_solve(np_ctx *ctx, np_batch *b, const np_sched *sched, np_stats *stats);
 /* Host copy of instance i's distances (int32[n_i]). */
 int np_batch_get(np_ctx *ctx, np_batch *b, int32_t i, int32_t *dist_out);
-/* Device address of instance i's distances. */
+/* Device address of instance i's distances, in the caller's vertex ids (valid until the
+ * next solve or free of the batch). */
 int np_batch_device_dist(np_batch *b, int32_t i, const int32_t **d_dist);
+/* Number of packs and of instances in the concatenated part. */
+int np_batch_layout(const np_batch *b, int32_t *packs, int32_t *concatenated);
 int np_batch_free(np_batch *b);
 
 /* Work counted for throughput: n_r = vertices with a finite distance, m_r = sum of
@@ -284,6 +301,9 @@ int np_read_graph_stream(int32_t fmt, void *fp, int32_t *n, int64_t *m, int32_t 
  * from `base`).  ms_out (optional, cap entries) receives each candidate's mean time. */
 int np_autotune(np_ctx *ctx, np_graph *g, const int32_t *srcs, int32_t nsrc, const np_sched *base,
                 np_sched *best, double *best_ms);
+/* The same search on a batch: each candidate times one np_batch_solve (all parts);
+ * *best_ms = the fastest solve.  The batch holds the last candidate's results after. */
+int np_batch_autotune(np_ctx *ctx, np_batch *b, const np_sched *base, np_sched *best, double *best_ms);
 
 #ifdef __cplusplus
 }

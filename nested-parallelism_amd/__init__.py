@@ -26,6 +26,7 @@ UNDEFINED = 0x7FFFFFFF    # apps/bfs-rec/src/gpu/bfs_rec.h:13
 NCLASS = 32
 MLP_NIN = 10
 CTX_RELABEL = 1           # np_set_option key: 0 caller ids, 1 always renumber, 2 when skewed
+CTX_PACK = 2              # np_set_option key: 1 batch instances sharing a graph are source-packed
 MLP_NPARAM = 32 * 10 + 32 + 32 * 32 + 32 + 5 * 32 + 5
 
 LEVEL_WAVE, LEVEL_WORKGROUP, LEVEL_GRID, LEVEL_LEARNED = 0, 1, 2, -1
@@ -65,7 +66,8 @@ class Sched(ctypes.Structure):
 
 class Stats(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int32), ("status", ctypes.c_int32), ("kernel_ms", ctypes.c_double),
-                ("frontier_total", ctypes.c_int64), ("reserved", ctypes.c_int64 * 6)]
+                ("frontier_total", ctypes.c_int64), ("drive", ctypes.c_int32), ("pad", ctypes.c_int32),
+                ("reserved", ctypes.c_int64 * 5)]
 
     def __repr__(self):
         return "Stats(iterations=%d, kernel_ms=%.4f, frontier_total=%d)" % (
@@ -115,6 +117,7 @@ _SIGS = [
     ("np_batch_solve", ctypes.c_int, [_vp, _vp, ctypes.POINTER(Sched), ctypes.POINTER(Stats)]),
     ("np_batch_get", ctypes.c_int, [_vp, _vp, ctypes.c_int32, _vp]),
     ("np_batch_device_dist", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.POINTER(_vp)]),
+    ("np_batch_layout", ctypes.c_int, [_vp, _i32p, _i32p]),
     ("np_batch_free", ctypes.c_int, [_vp]),
     ("np_reached_work", ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64),
                                        ctypes.POINTER(ctypes.c_int64)]),
@@ -156,6 +159,8 @@ _SIGS = [
                                             ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
     ("np_autotune", ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int32, ctypes.POINTER(Sched), ctypes.POINTER(Sched),
                                    ctypes.POINTER(ctypes.c_double)]),
+    ("np_batch_autotune", ctypes.c_int, [_vp, _vp, ctypes.POINTER(Sched), ctypes.POINTER(Sched),
+                                         ctypes.POINTER(ctypes.c_double)]),
 ]
 EXPORTS = [s[0] for s in _SIGS]
 
@@ -542,6 +547,19 @@ class Batch:
         p = _vp()
         _check(load().np_batch_device_dist(self.h, i, ctypes.byref(p)))
         return p.value
+
+    def autotune(self, base=None):
+        """Fastest schedule for this batch (np_batch_autotune): (Sched, ms per solve)."""
+        best, ms = Sched(), ctypes.c_double()
+        _check(load().np_batch_autotune(self.ctx.h, self.h, ctypes.byref(base or Sched()), ctypes.byref(best),
+                                        ctypes.byref(ms)))
+        return best, ms.value
+
+    def layout(self):
+        """(number of source-packed parts, instances in the concatenated part)."""
+        a, b = ctypes.c_int32(), ctypes.c_int32()
+        _check(load().np_batch_layout(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
 
     def free(self):
         if self.h:

@@ -73,6 +73,8 @@ struct Space {
     int2 *inst = nullptr;
     int *src_slots = nullptr;
     int src_cap = 0;
+    unsigned *mflag = nullptr;    // source-packed: pending sources per vertex
+    unsigned *qmask[2] = {nullptr, nullptr};
     // captured chunk of the graph drive
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
@@ -87,6 +89,7 @@ struct Space {
         if (own_cls) dfree(cls);
         cls = nullptr;
         dfree(lut); dfree(inst); dfree(src_slots);
+        dfree(mflag); dfree(qmask[0]); dfree(qmask[1]);
         drop_graph();
     }
     void drop_graph() {
@@ -118,6 +121,7 @@ struct np_ctx {
     unsigned *snap_bm = nullptr;
     long long snap_n = 0;
     int relabel = 2;  // NP_CTX_RELABEL: 0 never, 1 always, 2 when in-degree is skewed
+    int pack = 1;     // NP_CTX_PACK: batch instances sharing a graph go to source-packed states
 };
 
 struct np_graph {
@@ -145,16 +149,32 @@ struct np_graph {
     }
 };
 
+// Up to PACK_MAX sources of one graph in one source-packed state.
+struct Pack {
+    np_graph *g = nullptr;
+    int kshift = 0;          // dist rows of 2^kshift sources
+    std::vector<int> inst;   // batch instance of each lane
+    Space sp;
+};
+
 struct np_batch {
     np_ctx *ctx = nullptr;
     int count = 0;
     int unit = 0;
-    std::vector<int> base, nverts;
-    std::vector<np_graph *> graphs;                    // graph of each instance
+    // per instance
+    std::vector<int> nverts;
+    std::vector<np_graph *> graphs;
+    std::vector<int> pack_of, lane_of;  // pack and lane, or -1 (concatenated part)
+    std::vector<int> base;              // slot base in the concatenated part, or -1
+    std::vector<long long> out_off;     // offset of the instance's result in `out`
+    std::vector<Pack *> packs;
+    // concatenated part
+    int nconcat = 0;
     int *row = nullptr, *col = nullptr, *w = nullptr;  // concatenated graphs (or borrowed)
     bool owns_graph = false;
     Space sp;
-    int *out = nullptr;                                // distances mapped back to original ids
+    int *out = nullptr;                 // results in original ids (unpacked / mapped back)
+    long long out_total = 0;
 };
 
 // ------------------------------------------------------------------------------------
@@ -216,6 +236,7 @@ static DevState make_state(np_ctx *c, Space &s, const int *row, const int *col, 
     st.nwords = s.nwords;
     st.inf = inf;
     st.ctrl = c->ctrl;
+    st.kshift = -1;
     return st;
 }
 
@@ -254,8 +275,8 @@ static SchedDev make_schedd(np_ctx *c, const np_sched &s, int n_feat) {
     return d;
 }
 
-static int persistent_wgs(np_ctx *c, const np_sched &s, bool batch, bool unit) {
-    int occ = persistent_max_wgs_per_cu(batch, unit, s.frontier == NP_FRONTIER_PUSH);
+static int persistent_wgs(np_ctx *c, const np_sched &s, bool batch, bool unit, bool packed) {
+    int occ = persistent_max_wgs_per_cu(batch, unit, s.frontier == NP_FRONTIER_PUSH, packed);
     if (occ < 1) occ = 1;
     int per = s.wgs_per_cu > 0 ? std::min(s.wgs_per_cu, occ) : occ;
     return per * c->num_cus;
@@ -271,13 +292,24 @@ static int run_drive(np_ctx *c, Space &s, const DevState &st, const np_sched &sc
                      bool batch, bool unit, np_stats *stats) {
     int rc = NP_OK;
     float ms = 0.f;
-    if (sch.drive == NP_DRIVE_PERSISTENT) {
-        int wgs = persistent_wgs(c, sch, batch, unit);
+    int drive = sch.drive;
+    if (drive == NP_DRIVE_PERSISTENT) {
+        int wgs = persistent_wgs(c, sch, batch, unit, st.kshift >= 0);
         HIPCHK(hipEventRecord(c->ev0, c->stream));
-        HIPCHK(launch_persistent(st, sd, batch, unit, wgs, c->stream));
-        HIPCHK(hipEventRecord(c->ev1, c->stream));
-        if ((rc = read_ctrl(c))) return rc;
-    } else if (sch.drive == NP_DRIVE_HOSTLOOP) {
+        hipError_t e = launch_persistent(st, sd, batch, unit, wgs, c->stream);
+        if (e == hipErrorIllegalState || e == hipErrorCooperativeLaunchTooLarge) {
+            // No cooperative queue for this process (another process on the device holds
+            // the cooperative dispatch resource) or the grid no longer fits: nothing was
+            // enqueued, so the same state is driven one kernel per step instead.
+            (void)hipGetLastError();
+            drive = NP_DRIVE_HOSTLOOP;
+        } else {
+            if (e != hipSuccess) return hip_fail(e, "persistent launch", __LINE__);
+            HIPCHK(hipEventRecord(c->ev1, c->stream));
+            if ((rc = read_ctrl(c))) return rc;
+        }
+    }
+    if (drive == NP_DRIVE_HOSTLOOP) {
         const int wgs = c->num_cus * 8;
         HIPCHK(hipEventRecord(c->ev0, c->stream));
         for (int it = 0;; ++it) {
@@ -290,7 +322,7 @@ static int run_drive(np_ctx *c, Space &s, const DevState &st, const np_sched &sc
         }
         HIPCHK(hipEventRecord(c->ev1, c->stream));
         if ((rc = read_ctrl(c))) return rc;
-    } else {
+    } else if (drive == NP_DRIVE_GRAPH) {
         const int K = sch.graph_chunk > 0 ? sch.graph_chunk : 16;
         const int wgs = c->num_cus * 8;
         bool same = s.gexec && s.gchunk == K && s.gwgs == wgs && s.gbatch == batch && s.gunit == unit &&
@@ -343,6 +375,7 @@ static int run_drive(np_ctx *c, Space &s, const DevState &st, const np_sched &sc
         stats->iterations = h.iters;
         stats->status = rc;
         stats->kernel_ms = ms;
+        stats->drive = drive;
         stats->frontier_total = h.frontier_total;
     }
     return rc;
@@ -513,6 +546,10 @@ int np_set_option(np_ctx *c, int32_t key, int32_t value) {
         if (value < 0 || value > 2) { set_error("NP_CTX_RELABEL takes 0, 1 or 2"); return NP_ERR_ARG; }
         c->relabel = value;
         return NP_OK;
+    case NP_CTX_PACK:
+        if (value < 0 || value > 1) { set_error("NP_CTX_PACK takes 0 or 1"); return NP_ERR_ARG; }
+        c->pack = value;
+        return NP_OK;
     default:
         set_error("unknown option");
         return NP_ERR_ARG;
@@ -542,7 +579,7 @@ int np_destroy(np_ctx *c) {
 int np_device_info(np_ctx *c, int32_t *num_cus, int32_t *max_wgs_per_cu) {
     if (!c) { set_error("null context"); return NP_ERR_ARG; }
     if (num_cus) *num_cus = c->num_cus;
-    if (max_wgs_per_cu) *max_wgs_per_cu = persistent_max_wgs_per_cu(false, false, false);
+    if (max_wgs_per_cu) *max_wgs_per_cu = persistent_max_wgs_per_cu(false, false, false, false);
     return NP_OK;
 }
 
@@ -696,27 +733,57 @@ int np_trace_get(np_ctx *c, np_trace_rec *recs, int32_t cap, int32_t *count) {
 }
 
 // ---- batches -----------------------------------------------------------------------
-int np_batch_create(np_ctx *c, int32_t count, np_graph *const *graphs, const int32_t *srcs, int32_t unit,
-                    np_batch **out) {
-    if (!c || !graphs || !srcs || !out || count < 1) { set_error("bad batch arguments"); return NP_ERR_ARG; }
-    *out = nullptr;
-    HIPCHK(hipSetDevice(c->device));
+// A batch holds two kinds of parts:
+//   packs         2..32 sources of one graph in one source-packed state (dist rows of
+//                 2^kshift sources per vertex), one solve per pack
+//   concatenated  every other instance: graphs laid end to end in one slot space, one
+//                 solve for all of them
+// Instances of a graph that appears more than once go to packs (NP_CTX_PACK, default on).
+static int build_pack(np_ctx *c, np_batch *b, np_graph *g, const std::vector<int> &ids, const int32_t *srcs) {
+    Pack *pk = new Pack();
+    b->packs.push_back(pk);
+    pk->g = g;
+    pk->inst = ids;
+    const int k = (int)ids.size();
+    int ks = 0;
+    while ((1 << ks) < k) ++ks;
+    pk->kshift = ks;
+    Space &s = pk->sp;
+    s.nslots = g->n;
+    std::memcpy(s.hist, g->hist, sizeof(s.hist));
+    s.cls = g->cls;
+    s.own_cls = false;
+    int rc = space_alloc(c, s, false);
+    if (rc) return rc;
+    HIPCHK(dalloc(&s.dist, (size_t)g->n << ks));
+    HIPCHK(dalloc(&s.mflag, (size_t)g->n));
+    HIPCHK(dalloc(&s.qmask[0], (size_t)s.qcap));
+    HIPCHK(dalloc(&s.qmask[1], (size_t)s.qcap));
+    if ((rc = ensure_src(c, s, k))) return rc;
+    std::vector<int> sid(k);
+    for (int j = 0; j < k; ++j) {
+        sid[j] = g->solver_id(srcs[ids[j]], c->stream);
+        if (sid[j] < 0) { set_error("source mapping failed"); return NP_ERR_HIP; }
+        b->pack_of[ids[j]] = (int)b->packs.size() - 1;
+        b->lane_of[ids[j]] = j;
+    }
+    HIPCHK(hipMemcpyAsync(s.src_slots, sid.data(), sizeof(int) * (size_t)k, hipMemcpyHostToDevice, c->stream));
+    return ctx_sync(c);
+}
+
+static int build_concat(np_ctx *c, np_batch *b, const std::vector<int> &ids, np_graph *const *graphs,
+                        const int32_t *srcs) {
     std::vector<np_graph *> uniq;
-    std::vector<int> gi(count);
-    for (int i = 0; i < count; ++i) {
-        np_graph *g = graphs[i];
-        if (!g || g->ctx != c) { set_error("instance %d: bad graph", i); return NP_ERR_ARG; }
-        if (srcs[i] < 0 || srcs[i] >= g->n) { set_error("instance %d: source out of range", i); return NP_ERR_ARG; }
+    std::vector<int> gi(ids.size());
+    for (size_t t = 0; t < ids.size(); ++t) {
+        np_graph *g = graphs[ids[t]];
         int k = -1;
         for (size_t j = 0; j < uniq.size(); ++j) if (uniq[j] == g) k = (int)j;
         if (k < 0) { k = (int)uniq.size(); uniq.push_back(g); }
-        gi[i] = k;
+        gi[t] = k;
     }
-    np_batch *b = new np_batch();
-    b->ctx = c; b->count = count; b->unit = unit ? 1 : 0;
     Space &s = b->sp;
     std::vector<long long> VB(uniq.size()), EB(uniq.size());
-    int rc = NP_OK;
     // every graph runs in its solver layout (renumbered when skewed)
     if (uniq.size() == 1) {
         b->row = const_cast<int *>(uniq[0]->srow); b->col = const_cast<int *>(uniq[0]->scol);
@@ -726,58 +793,103 @@ int np_batch_create(np_ctx *c, int32_t count, np_graph *const *graphs, const int
     } else {
         long long nv = 0, ne = 0;
         for (size_t j = 0; j < uniq.size(); ++j) { VB[j] = nv; EB[j] = ne; nv += uniq[j]->n + 1; ne += uniq[j]->m; }
-        if (ne > 0x7FFFFFFFll || nv > 0x7FFFFFFFll) { delete b; set_error("batch exceeds 2^31 edges"); return NP_ERR_CAPACITY; }
+        if (ne > 0x7FFFFFFFll || nv > 0x7FFFFFFFll) { set_error("batch exceeds 2^31 edges"); return NP_ERR_CAPACITY; }
         b->owns_graph = true;
-        hipError_t e = dalloc(&b->row, (size_t)nv);
-        if (e == hipSuccess) e = dalloc(&b->col, (size_t)ne);
-        if (e == hipSuccess) e = dalloc(&b->w, (size_t)ne);
-        for (size_t j = 0; j < uniq.size() && e == hipSuccess; ++j) {
+        HIPCHK(dalloc(&b->row, (size_t)nv));
+        HIPCHK(dalloc(&b->col, (size_t)ne));
+        HIPCHK(dalloc(&b->w, (size_t)ne));
+        for (size_t j = 0; j < uniq.size(); ++j) {
             np_graph *g = uniq[j];
-            e = launch_rebase_row(g->srow, g->n, (int)EB[j], b->row + VB[j], c->stream);
-            if (e == hipSuccess && g->m)
-                e = hipMemcpyAsync(b->col + EB[j], g->scol, sizeof(int) * (size_t)g->m, hipMemcpyDeviceToDevice, c->stream);
-            if (e == hipSuccess && g->m)
-                e = hipMemcpyAsync(b->w + EB[j], g->sw, sizeof(int) * (size_t)g->m, hipMemcpyDeviceToDevice, c->stream);
+            HIPCHK(launch_rebase_row(g->srow, g->n, (int)EB[j], b->row + VB[j], c->stream));
+            if (g->m) {
+                HIPCHK(hipMemcpyAsync(b->col + EB[j], g->scol, sizeof(int) * (size_t)g->m, hipMemcpyDeviceToDevice, c->stream));
+                HIPCHK(hipMemcpyAsync(b->w + EB[j], g->sw, sizeof(int) * (size_t)g->m, hipMemcpyDeviceToDevice, c->stream));
+            }
         }
-        if (e != hipSuccess) { rc = hip_fail(e, "batch graph concat", __LINE__); np_batch_free(b); return rc; }
     }
     // slots: instance i owns [base_i, base_i + pad(n_i)), pad to 2^LUT_SHIFT
     const long long G = 1ll << LUT_SHIFT;
+    const int cnt = (int)ids.size();
     long long acc = 0;
-    b->base.resize(count); b->nverts.resize(count); b->graphs.resize(count);
-    std::vector<int2> inst(count);
-    for (int i = 0; i < count; ++i) {
-        np_graph *g = uniq[gi[i]];
-        b->base[i] = (int)acc; b->nverts[i] = g->n; b->graphs[i] = g;
-        inst[i] = make_int2((int)acc, (int)VB[gi[i]]);
+    std::vector<int2> inst(cnt);
+    for (int t = 0; t < cnt; ++t) {
+        np_graph *g = uniq[gi[t]];
+        b->base[ids[t]] = (int)acc;
+        inst[t] = make_int2((int)acc, (int)VB[gi[t]]);
         for (int k = 0; k < NCLS; ++k) s.hist[k] += g->hist[k];
         acc += ((long long)g->n + G - 1) / G * G;
-        if (acc > 0x7FFFFFFFll - G) { np_batch_free(b); set_error("batch exceeds 2^31 vertex slots"); return NP_ERR_CAPACITY; }
+        if (acc > 0x7FFFFFFFll - G) { set_error("batch exceeds 2^31 vertex slots"); return NP_ERR_CAPACITY; }
     }
     s.nslots = acc > 0 ? acc : G;
-    if ((rc = space_alloc(c, s, true))) { np_batch_free(b); return rc; }
-    hipError_t e = dalloc(&s.cls, (size_t)s.nslots);
+    int rc = space_alloc(c, s, true);
+    if (rc) return rc;
+    HIPCHK(dalloc(&s.cls, (size_t)s.nslots));
     s.own_cls = true;
-    if (e == hipSuccess) e = dalloc(&s.lut, (size_t)(s.nslots / G + 1));
-    if (e == hipSuccess) e = dalloc(&s.inst, (size_t)count);
-    if (e == hipSuccess) e = launch_fill_u32((unsigned *)s.cls, (s.nslots + 3) / 4, 0u, c->stream);
-    for (int i = 0; i < count && e == hipSuccess; ++i) {
-        np_graph *g = uniq[gi[i]];
-        e = launch_copy_cls(g->cls, s.cls + b->base[i], g->n, c->stream);
-        if (e == hipSuccess) e = launch_fill_lut(s.lut, b->base[i] / G, ((long long)g->n + G - 1) / G, i, c->stream);
+    HIPCHK(dalloc(&s.lut, (size_t)(s.nslots / G + 1)));
+    HIPCHK(dalloc(&s.inst, (size_t)cnt));
+    HIPCHK(launch_fill_u32((unsigned *)s.cls, (s.nslots + 3) / 4, 0u, c->stream));
+    for (int t = 0; t < cnt; ++t) {
+        np_graph *g = uniq[gi[t]];
+        HIPCHK(launch_copy_cls(g->cls, s.cls + b->base[ids[t]], g->n, c->stream));
+        HIPCHK(launch_fill_lut(s.lut, b->base[ids[t]] / G, ((long long)g->n + G - 1) / G, t, c->stream));
     }
-    if (e == hipSuccess) e = hipMemcpyAsync(s.inst, inst.data(), sizeof(int2) * (size_t)count, hipMemcpyHostToDevice, c->stream);
-    if (e != hipSuccess) { rc = hip_fail(e, "batch setup", __LINE__); np_batch_free(b); return rc; }
-    if ((rc = ensure_src(c, s, count))) { np_batch_free(b); return rc; }
-    std::vector<int> slots(count);
+    HIPCHK(hipMemcpyAsync(s.inst, inst.data(), sizeof(int2) * (size_t)cnt, hipMemcpyHostToDevice, c->stream));
+    if ((rc = ensure_src(c, s, cnt))) return rc;
+    std::vector<int> slots(cnt);
+    for (int t = 0; t < cnt; ++t) {
+        const int sid = graphs[ids[t]]->solver_id(srcs[ids[t]], c->stream);
+        if (sid < 0) { set_error("source mapping failed"); return NP_ERR_HIP; }
+        slots[t] = b->base[ids[t]] + sid;
+    }
+    HIPCHK(hipMemcpyAsync(s.src_slots, slots.data(), sizeof(int) * (size_t)cnt, hipMemcpyHostToDevice, c->stream));
+    b->nconcat = cnt;
+    return ctx_sync(c);
+}
+
+int np_batch_create(np_ctx *c, int32_t count, np_graph *const *graphs, const int32_t *srcs, int32_t unit,
+                    np_batch **out) {
+    if (!c || !graphs || !srcs || !out || count < 1) { set_error("bad batch arguments"); return NP_ERR_ARG; }
+    *out = nullptr;
+    HIPCHK(hipSetDevice(c->device));
+    std::vector<np_graph *> uniq;
+    std::vector<std::vector<int>> by_graph;
     for (int i = 0; i < count; ++i) {
-        const int sid = b->graphs[i]->solver_id(srcs[i], c->stream);
-        if (sid < 0) { np_batch_free(b); set_error("source mapping failed"); return NP_ERR_HIP; }
-        slots[i] = b->base[i] + sid;
+        np_graph *g = graphs[i];
+        if (!g || g->ctx != c) { set_error("instance %d: bad graph", i); return NP_ERR_ARG; }
+        if (srcs[i] < 0 || srcs[i] >= g->n) { set_error("instance %d: source out of range", i); return NP_ERR_ARG; }
+        int k = -1;
+        for (size_t j = 0; j < uniq.size(); ++j) if (uniq[j] == g) k = (int)j;
+        if (k < 0) { k = (int)uniq.size(); uniq.push_back(g); by_graph.emplace_back(); }
+        by_graph[k].push_back(i);
     }
-    e = hipMemcpyAsync(s.src_slots, slots.data(), sizeof(int) * (size_t)count, hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    if (e != hipSuccess) { rc = hip_fail(e, "batch sources", __LINE__); np_batch_free(b); return rc; }
+    np_batch *b = new np_batch();
+    b->ctx = c; b->count = count; b->unit = unit ? 1 : 0;
+    b->nverts.resize(count); b->graphs.resize(count); b->base.assign(count, -1);
+    b->pack_of.assign(count, -1); b->lane_of.assign(count, -1); b->out_off.resize(count);
+    long long off = 0;
+    for (int i = 0; i < count; ++i) {
+        b->graphs[i] = graphs[i]; b->nverts[i] = graphs[i]->n;
+        b->out_off[i] = off; off += graphs[i]->n;
+    }
+    b->out_total = off;
+    std::vector<int> rest;
+    int rc = NP_OK;
+    for (size_t j = 0; j < uniq.size() && rc == NP_OK; ++j) {
+        const std::vector<int> &ids = by_graph[j];
+        const int k = (int)ids.size();
+        if (!c->pack || k < 2) { rest.insert(rest.end(), ids.begin(), ids.end()); continue; }
+        const int npk = (k + PACK_MAX - 1) / PACK_MAX;  // packs of near-equal size
+        for (int p = 0, at = 0; p < npk && rc == NP_OK; ++p) {
+            const int sz = k / npk + (p < k % npk ? 1 : 0);
+            rc = build_pack(c, b, uniq[j], std::vector<int>(ids.begin() + at, ids.begin() + at + sz), srcs);
+            at += sz;
+        }
+    }
+    if (rc == NP_OK && !rest.empty()) {
+        std::sort(rest.begin(), rest.end());
+        rc = build_concat(c, b, rest, graphs, srcs);
+    }
+    if (rc) { np_batch_free(b); return rc; }
     *out = b;
     return NP_OK;
 }
@@ -787,27 +899,70 @@ int np_batch_solve(np_ctx *c, np_batch *b, const np_sched *sched, np_stats *stat
     np_sched sch;
     int rc = validate_sched(sched, &sch);
     if (rc) return rc;
-    Space &s = b->sp;
     const bool unit = b->unit != 0;
-    DevState st = make_state(c, s, b->row, b->col, unit ? nullptr : b->w, s.dist, unit ? UNDEFINED : INF);
-    st.cached_filter = (sch.options & NP_OPT_CACHED_FILTER) ? 1 : 0;
-    int nmax = 1;
-    for (int i = 0; i < b->count; ++i) nmax = std::max(nmax, b->nverts[i]);
-    SchedDev sd = make_schedd(c, sch, nmax);
-    HIPCHK(launch_init_solve(st, s.src_slots, b->count, sd.push, sd.delta, c->stream));
-    return run_drive(c, s, st, sch, sd, true, unit, stats);
+    const int inf = unit ? UNDEFINED : INF;
+    np_stats acc;
+    std::memset(&acc, 0, sizeof acc);
+    acc.drive = -1;
+    auto add = [&](const np_stats &p) {
+        acc.iterations += p.iterations;
+        acc.kernel_ms += p.kernel_ms;
+        acc.frontier_total += p.frontier_total;
+        if (acc.drive < 0) acc.drive = p.drive;
+    };
+    // packs: scan frontier, no priority window
+    np_sched psch = sch;
+    psch.frontier = NP_FRONTIER_SCAN;
+    psch.delta = 0;
+    for (Pack *pk : b->packs) {
+        np_graph *g = pk->g;
+        Space &s = pk->sp;
+        DevState st = make_state(c, s, g->srow, g->scol, unit ? nullptr : g->sw, s.dist, inf);
+        st.cached_filter = (sch.options & NP_OPT_CACHED_FILTER) ? 1 : 0;
+        st.kshift = pk->kshift;
+        st.mflag = s.mflag;
+        st.qmask[0] = s.qmask[0]; st.qmask[1] = s.qmask[1];
+        SchedDev sd = make_schedd(c, psch, g->n);
+        HIPCHK(launch_init_packed(st, s.src_slots, (int)pk->inst.size(), c->stream));
+        np_stats ps;
+        if ((rc = run_drive(c, s, st, psch, sd, false, unit, &ps))) return rc;
+        add(ps);
+    }
+    if (b->nconcat > 0) {
+        Space &s = b->sp;
+        DevState st = make_state(c, s, b->row, b->col, unit ? nullptr : b->w, s.dist, inf);
+        st.cached_filter = (sch.options & NP_OPT_CACHED_FILTER) ? 1 : 0;
+        int nmax = 1;
+        for (int i = 0; i < b->count; ++i) if (b->pack_of[i] < 0) nmax = std::max(nmax, b->nverts[i]);
+        SchedDev sd = make_schedd(c, sch, nmax);
+        HIPCHK(launch_init_solve(st, s.src_slots, b->nconcat, sd.push, sd.delta, c->stream));
+        np_stats ps;
+        if ((rc = run_drive(c, s, st, sch, sd, true, unit, &ps))) return rc;
+        add(ps);
+    }
+    if (stats) *stats = acc;
+    return NP_OK;
 }
 
-// Instance i's distances in original vertex ids (mapped back when its graph is renumbered).
+// Instance i's distances in original vertex ids: unpacked from its pack, or mapped back
+// when its graph is renumbered, into the batch's result buffer.
 static int batch_result(np_ctx *c, np_batch *b, int i, const int **res) {
     const np_graph *g = b->graphs[i];
-    const int *d = b->sp.dist + b->base[i];
-    if (g->rl.active) {
-        if (!b->out) HIPCHK(dalloc(&b->out, (size_t)b->sp.nslots));
-        HIPCHK(launch_unpermute(d, g->rl.old2new, g->n, b->out + b->base[i], c->stream));
-        d = b->out + b->base[i];
+    const int *old2new = g->rl.active ? g->rl.old2new : nullptr;
+    const int pk = b->pack_of[i];
+    if (pk < 0 && !old2new) {
+        *res = b->sp.dist + b->base[i];
+        return NP_OK;
     }
-    *res = d;
+    if (!b->out) HIPCHK(dalloc(&b->out, (size_t)b->out_total));
+    int *dst = b->out + b->out_off[i];
+    if (pk >= 0) {
+        const Pack *p = b->packs[pk];
+        HIPCHK(launch_unpack(p->sp.dist, p->kshift, b->lane_of[i], old2new, g->n, dst, c->stream));
+    } else {
+        HIPCHK(launch_unpermute(b->sp.dist + b->base[i], old2new, g->n, dst, c->stream));
+    }
+    *res = dst;
     return NP_OK;
 }
 
@@ -827,10 +982,19 @@ int np_batch_device_dist(np_batch *b, int32_t i, const int32_t **d_dist) {
     return ctx_sync(b->ctx);
 }
 
+int np_batch_layout(const np_batch *b, int32_t *packs, int32_t *concatenated) {
+    if (!b) { set_error("null batch"); return NP_ERR_ARG; }
+    if (packs) *packs = (int32_t)b->packs.size();
+    if (concatenated) *concatenated = b->nconcat;
+    return NP_OK;
+}
+
 int np_batch_free(np_batch *b) {
     if (!b) return NP_OK;
     if (b->ctx) (void)hipSetDevice(b->ctx->device);
     b->sp.release();
+    for (Pack *pk : b->packs) { pk->sp.release(); delete pk; }
+    b->packs.clear();
     if (b->owns_graph) { dfree(b->row); dfree(b->col); dfree(b->w); }
     dfree(b->out);
     delete b;
@@ -1072,7 +1236,7 @@ int np_autotune(np_ctx *c, np_graph *g, const int32_t *srcs, int32_t nsrc, const
     np_sched b0;
     int rc = validate_sched(base, &b0);
     if (rc) return rc;
-    int occ = persistent_max_wgs_per_cu(false, false, b0.frontier == NP_FRONTIER_PUSH);
+    int occ = persistent_max_wgs_per_cu(false, false, b0.frontier == NP_FRONTIER_PUSH, false);
     std::vector<int> wgs_opts = {0};
     if (occ > 2) wgs_opts.push_back(2);
     if (occ > 4) wgs_opts.push_back(4);
@@ -1105,6 +1269,38 @@ int np_autotune(np_ctx *c, np_graph *g, const int32_t *srcs, int32_t nsrc, const
     dfree(d_dist);
     *best = bs;
     if (best_ms) *best_ms = bestt / nsrc;
+    return NP_OK;
+}
+
+int np_batch_autotune(np_ctx *c, np_batch *b, const np_sched *base, np_sched *best, double *best_ms) {
+    if (!c || !b || b->ctx != c || !best) { set_error("bad arguments"); return NP_ERR_ARG; }
+    np_sched b0;
+    int rc = validate_sched(base, &b0);
+    if (rc) return rc;
+    int occ = persistent_max_wgs_per_cu(true, b->unit != 0, false, !b->packs.empty());
+    std::vector<int> wgs_opts = {0};
+    if (occ > 2) wgs_opts.push_back(2);
+    if (occ > 4) wgs_opts.push_back(4);
+    const int klos[] = {3, 4, 5, 6, 7};
+    const int khis[] = {6, 8, 10, 12, 14, 30};
+    double bestt = 1e30;
+    np_sched bs = b0;
+    for (int lv = 0; lv <= 2; ++lv)
+        for (int klo : klos)
+            for (int khi : khis) {
+                if (lv == 0 && khi != khis[0]) continue;  // level 0 has no upper bin
+                if (khi < klo) continue;
+                for (int wg : wgs_opts) {
+                    np_sched s = b0;
+                    s.level = lv; s.log2_t_lo = klo; s.log2_t_hi = khi; s.wgs_per_cu = wg;
+                    s.trace = 0;
+                    np_stats st;
+                    if ((rc = np_batch_solve(c, b, &s, &st))) return rc;
+                    if (st.kernel_ms < bestt) { bestt = st.kernel_ms; bs = s; }
+                }
+            }
+    *best = bs;
+    if (best_ms) *best_ms = bestt;
     return NP_OK;
 }
 

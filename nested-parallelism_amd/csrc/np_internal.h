@@ -47,7 +47,16 @@ struct DevState {
     int inf;                  // unreached value written at init
     int cached_filter;        // NP_OPT_CACHED_FILTER
     Ctrl *ctrl;
+    // source-packed batch (kshift >= 0): 2^kshift sources of one graph per vertex row,
+    // dist[(v << kshift) + i]; mflag[v] = sources pending at v; bm[0] = vertices with a
+    // nonzero mflag; qmask[] = source mask of each queue entry
+    int kshift;               // -1: one distance per slot
+    unsigned *mflag;
+    unsigned *qmask[2];
 };
+
+// Largest number of sources packed into one distance row.
+constexpr int PACK_MAX = 32;
 
 struct SchedDev {
     int learned;            // evaluate the MLP each iteration
@@ -77,10 +86,17 @@ hipError_t launch_fill_lut(int *lut, long long first, long long count, int inst,
 hipError_t launch_copy_cls(const unsigned char *src, unsigned char *dst, long long n,
                            hipStream_t s);
 
-// Persistent drive: one cooperative launch.  `wgs` workgroups of 256 threads.
+// Source-packed state: dist rows set to inf, source i of srcs[] at lane i (nsrc <= 2^kshift).
+hipError_t launch_init_packed(const DevState &st, const int *d_srcs, int nsrc, hipStream_t s);
+// out[v] = dist[(map(v) << kshift) + lane], map = old2new or identity (old2new null).
+hipError_t launch_unpack(const int *dist, int kshift, int lane, const int *old2new, int n, int *out,
+                         hipStream_t s);
+
+// Persistent drive: one cooperative launch.  `wgs` workgroups of 256 threads.  A state
+// with kshift >= 0 runs the source-packed kernels (here and in compact/expand).
 hipError_t launch_persistent(const DevState &st, const SchedDev &sp, bool batch, bool unit,
                              int wgs, hipStream_t s);
-int persistent_max_wgs_per_cu(bool batch, bool unit, bool push);
+int persistent_max_wgs_per_cu(bool batch, bool unit, bool push, bool packed);
 
 // Host-loop / graph drive, one iteration = [compact] + decide + expand.
 hipError_t launch_compact(const DevState &st, int it_off, bool batch, int wgs, hipStream_t s);

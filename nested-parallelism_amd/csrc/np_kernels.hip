@@ -660,6 +660,335 @@ __global__ void k_set_cfg(DevState st, int level, int klo, int khi) {
 }
 
 // ------------------------------------------------------------------------------------
+// source-packed batch: up to 32 sources of one graph solved together
+// ------------------------------------------------------------------------------------
+// Distances are vertex-major, 2^kshift sources per vertex row: dist[(v << kshift) + i].
+// The distances of every source at one vertex share a cache line (64 B for 16 sources),
+// so one read of an edge's col/w serves every source pending at its tail, and the check
+// of dist[v][*] is one coalesced line instead of one random sector per source.
+// mflag[v] collects the sources whose distance at v improved since v was last queued;
+// bm[0] has one bit per vertex with a nonzero mflag (set by the relaxation that made it
+// nonzero), so the compaction scans n/8 bytes and reads mflag only where work is pending.
+// A queue entry carries the source mask it was queued with (qmask).  The work of a
+// queued vertex is out-degree x pending sources; its items are numbered edge-major,
+// item t = (edge t / P, t % P-th pending source), so the P lanes of one edge are adjacent.
+// The relaxation per (edge, source) is the reference's: alt = dist[u][i] + w;
+// if (dist[v][i] > alt) atomicMin -- int32, as in the single-source kernels.
+
+// Position of the k-th (0-based) set bit of m (k < popcount(m)).
+__device__ __forceinline__ int nth_set_bit(unsigned m, int k) {
+    int pos = 0, c;
+    c = __popc(m & 0xFFFFu); if (k >= c) { k -= c; m >>= 16; pos += 16; }
+    c = __popc(m & 0xFFu);   if (k >= c) { k -= c; m >>= 8;  pos += 8; }
+    c = __popc(m & 0xFu);    if (k >= c) { k -= c; m >>= 4;  pos += 4; }
+    c = __popc(m & 0x3u);    if (k >= c) { k -= c; m >>= 2;  pos += 2; }
+    c = (int)(m & 1u);       if (k >= c) { pos += 1; }
+    return pos;
+}
+
+// Lane-bin vertices of a packed expansion have out-degree < 2^(PACK_LANE_CLS - 1), so a
+// 64-vertex tile holds < 2^27 items (int) and item / P is exact in float arithmetic.
+constexpr int PACK_LANE_CLS = 17;
+
+__device__ __forceinline__ void pk_relax_tail(const DevState &st, int v, int src, int alt, int *p) {
+    const int old = atomicMin(p, alt);
+    if (alt >= old || st.cls[v] == 0) return;  // no improvement, or nothing to propagate
+    const unsigned prev = atomicOr(&st.mflag[v], 1u << src);
+    if (prev == 0) atomicOr(&st.bm[0][v >> 5], 1u << (v & 31));
+}
+
+// Items of one queued vertex u, edges [e0, e1) x pending sources M; worker t of `stride`.
+template <bool UNIT>
+__device__ __forceinline__ void pk_relax_vertex(const DevState &st, int u, unsigned M, int e0, int e1,
+                                                int t, int stride) {
+    const int P = __popc(M);
+    if (P == 0 || e0 >= e1) return;
+    const int ks = st.kshift;
+    const int qs = stride / P, rs = stride - qs * P;  // one step of `stride` items
+    int e = e0 + t / P, j = t - (t / P) * P;
+    const int *du_row = st.dist + ((long long)u << ks);
+    while (e < e1) {
+        int tv[UNR], alt[UNR], cur[UNR], src[UNR];
+        bool ok[UNR];
+#pragma unroll
+        for (int k = 0; k < UNR; ++k) {
+            ok[k] = e < e1;
+            const int ec = ok[k] ? e : e0;
+            src[k] = nth_set_bit(M, j);
+            tv[k] = st.col[ec];
+            const int du = ld_coherent(du_row + src[k]);
+            alt[k] = ok[k] ? du + (UNIT ? 1 : st.w[ec]) : 0;
+            e += qs; j += rs;
+            if (j >= P) { j -= P; e += 1; }
+        }
+#pragma unroll
+        for (int k = 0; k < UNR; ++k) {
+            const int *p = st.dist + ((long long)tv[k] << ks) + src[k];
+            cur[k] = ok[k] ? (st.cached_filter ? *p : ld_coherent(p)) : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < UNR; ++k)
+            if (ok[k] && alt[k] < cur[k])
+                pk_relax_tail(st, tv[k], src[k], alt[k], st.dist + ((long long)tv[k] << ks) + src[k]);
+    }
+}
+
+// Lane bin: 64 queued vertices, their items balanced over the lanes.
+template <bool UNIT>
+__device__ __forceinline__ void pk_lane_tile(const DevState &st, const int *q, const unsigned *qm,
+                                             long long qidx, int nvalid) {
+    const int lane = lane_id();
+    const int ks = st.kshift;
+    int u = 0, s = 0, P = 0, items = 0;
+    unsigned M = 0;
+    if (lane < nvalid) {
+        u = ld_coherent(q + qidx + lane);
+        M = ld_coherent_u(qm + qidx + lane);
+        s = st.row[u];
+        P = __popc(M);
+        items = (st.row[u + 1] - s) * P;
+    }
+    const float rcp = P ? 1.0f / (float)P : 0.0f;
+    const int incl = wave_incl_scan(items);
+    const int total = __shfl(incl, 63);
+    const int excl = incl - items;
+    if (total == 0) return;
+    const int first_lane = __ffsll((long long)__ballot(items > 0)) - 1;
+    const int e_safe = __shfl(s, first_lane);
+    for (int base = 0; base < total; base += UNR * 64) {
+        int tv[UNR], alt[UNR], cur[UNR], src[UNR];
+        bool ok[UNR];
+#pragma unroll
+        for (int k = 0; k < UNR; ++k) {
+            const int kk = base + lane + k * 64;
+            ok[k] = kk < total;
+            int o = 0;  // owner: the last lane whose exclusive offset is <= kk
+#pragma unroll
+            for (int step = 32; step > 0; step >>= 1) {
+                int x = __shfl(excl, o + step);
+                if (x <= kk) o += step;
+            }
+            const int r = kk - __shfl(excl, o);
+            const int Po = __shfl(P, o);
+            const float ro = __shfl(rcp, o);
+            const unsigned Mo = __shfl(M, o);
+            const int so = __shfl(s, o);
+            const int uo = __shfl(u, o);
+            int eo = (int)((float)r * ro);  // r / Po, then corrected by at most one
+            int jo = r - eo * Po;
+            if (jo < 0) { eo -= 1; jo += Po; }
+            else if (jo >= Po) { eo += 1; jo -= Po; }
+            const int ec = ok[k] ? so + eo : e_safe;
+            src[k] = ok[k] ? nth_set_bit(Mo, jo) : 0;
+            tv[k] = st.col[ec];
+            alt[k] = ok[k] ? ld_coherent(st.dist + ((long long)uo << ks) + src[k]) + (UNIT ? 1 : st.w[ec]) : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < UNR; ++k) {
+            const int *p = st.dist + ((long long)tv[k] << ks) + src[k];
+            cur[k] = ok[k] ? (st.cached_filter ? *p : ld_coherent(p)) : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < UNR; ++k)
+            if (ok[k] && alt[k] < cur[k])
+                pk_relax_tail(st, tv[k], src[k], alt[k], st.dist + ((long long)tv[k] << ks) + src[k]);
+    }
+}
+
+template <bool UNIT>
+__device__ void pk_expansion(const DevState &st, const int *q, const unsigned *qm, ItemTables &tb) {
+    const long long G = tb.pre[0][NCLS], K = tb.pre[1][NCLS], W = tb.pre[2][NCLS],
+                    T = tb.pre[3][NCLS];
+    for (long long k = blockIdx.x; k < K; k += gridDim.x) {
+        int c = find_class(tb.pre[1], k);
+        long long qi = tb.seg[c] + (k - tb.pre[1][c]);
+        int u = ld_coherent(q + qi);
+        unsigned M = ld_coherent_u(qm + qi);
+        pk_relax_vertex<UNIT>(st, u, M, st.row[u], st.row[u + 1], threadIdx.x, WG);
+    }
+    const long long NW = (long long)gridDim.x * WAVES;
+    const long long gw = (long long)blockIdx.x * WAVES + (threadIdx.x >> 6);
+    const long long U = G + W + T;
+    for (long long x = gw; x < U; x += NW) {
+        if (x < G) {
+            int c = find_class(tb.pre[0], x);
+            long long r = x - tb.pre[0][c];
+            int sh = c > GRID_CHUNK_SHIFT ? c - GRID_CHUNK_SHIFT : 0;
+            long long qi = tb.seg[c] + (r >> sh);
+            int j = (int)(r & ((1ll << sh) - 1));
+            int u = ld_coherent(q + qi);
+            unsigned M = ld_coherent_u(qm + qi);
+            int s = st.row[u], e = st.row[u + 1];
+            int e0 = s + (j << GRID_CHUNK_SHIFT);
+            int e1 = e0 + (1 << GRID_CHUNK_SHIFT);
+            if (e1 > e) e1 = e;
+            pk_relax_vertex<UNIT>(st, u, M, e0, e1, lane_id(), 64);
+        } else if (x < G + W) {
+            long long i = x - G;
+            int c = find_class(tb.pre[2], i);
+            long long qi = tb.seg[c] + (i - tb.pre[2][c]);
+            int u = ld_coherent(q + qi);
+            unsigned M = ld_coherent_u(qm + qi);
+            pk_relax_vertex<UNIT>(st, u, M, st.row[u], st.row[u + 1], lane_id(), 64);
+        } else {
+            long long t = x - G - W;
+            int c = find_class(tb.pre[3], t);
+            long long ti = t - tb.pre[3][c];
+            long long first = ti * 64;
+            int nvalid = (int)((long long)tb.cnt[c] - first < 64 ? (long long)tb.cnt[c] - first : 64);
+            pk_lane_tile<UNIT>(st, q, qm, tb.seg[c] + first, nvalid);
+        }
+    }
+}
+
+// Vertices with pending sources -> class-segmented queue with their source masks.
+__device__ __forceinline__ void pk_compaction(const DevState &st, int *cnt_out, int *q_out, unsigned *qm_out) {
+    __shared__ int s_lc[NCLS], s_lb[NCLS];
+    unsigned *bm = st.bm[0];
+    for (long long wb = (long long)blockIdx.x * WG; wb < st.nwords; wb += (long long)gridDim.x * WG) {
+        if (threadIdx.x < NCLS) s_lc[threadIdx.x] = 0;
+        __syncthreads();
+        const long long wi = wb + threadIdx.x;
+        const unsigned word = wi < st.nwords ? ld_coherent_u(bm + wi) : 0u;
+        for (unsigned w = word; w; w &= w - 1) {
+            const long long v = wi * 32 + (__ffs(w) - 1);
+            const int c = st.cls[v];
+            if (c) atomicAdd(&s_lc[c], 1);  // class 0 is never flagged; kept out regardless
+        }
+        __syncthreads();
+        if (threadIdx.x < NCLS) {
+            int k = s_lc[threadIdx.x];
+            s_lb[threadIdx.x] = k ? atomicAdd(&cnt_out[threadIdx.x], k) : 0;
+            s_lc[threadIdx.x] = 0;
+        }
+        __syncthreads();
+        for (unsigned w = word; w; w &= w - 1) {
+            const long long v = wi * 32 + (__ffs(w) - 1);
+            const int c = st.cls[v];
+            if (!c) continue;
+            const int p = atomicAdd(&s_lc[c], 1);
+            const long long qi = st.seg_off[c] + s_lb[c] + p;
+            st_coherent(q_out + qi, (int)v);
+            st_coherent_u(qm_out + qi, ld_coherent_u(st.mflag + v));
+            st_coherent_u(st.mflag + v, 0u);
+        }
+        if (word) st_coherent_u(bm + wi, 0u);
+        __syncthreads();
+    }
+}
+
+// Lane-bin bound of a packed expansion (see PACK_LANE_CLS); the other bins keep theirs.
+__device__ __forceinline__ Cfg pk_cfg(Cfg c) {
+    if (c.b1 > PACK_LANE_CLS) c.b1 = PACK_LANE_CLS;
+    return c;
+}
+
+template <bool UNIT>
+__global__ __launch_bounds__(WG) void k_pk_persistent(DevState st, SchedDev sp) {
+    __shared__ ItemTables tb;
+    __shared__ int s_total;
+    Ctrl *ctrl = st.ctrl;
+    unsigned long long gen = 0;
+    const unsigned nb = gridDim.x;
+    long long t_prev = 0;
+    for (int it = 0;; ++it) {
+        int *cnt_cur = st.cnt + (it % 3) * NCLS;
+        int *cnt_zero = st.cnt + ((it + 2) % 3) * NCLS;
+        int *q_cur = st.queue[it & 1];
+        unsigned *qm_cur = st.qmask[it & 1];
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            long long now = (long long)wall_clock64();
+            if (it > 0 && sp.trace && it - 1 < sp.trace_cap) sp.trace[it - 1].t_end = now;
+            t_prev = now;
+        }
+        if (blockIdx.x == 0 && threadIdx.x < NCLS) st_coherent(cnt_zero + threadIdx.x, 0);
+        pk_compaction(st, cnt_cur, q_cur, qm_cur);
+        if (!grid_sync(ctrl, nb, gen)) return;
+        if (threadIdx.x < NCLS) {
+            tb.cnt[threadIdx.x] = ld_coherent(cnt_cur + threadIdx.x);
+            tb.seg[threadIdx.x] = st.seg_off[threadIdx.x];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            long long tot = 0;
+            for (int c = 1; c < NCLS; ++c) tot += tb.cnt[c];
+            s_total = tot > 0x7FFFFFFF ? 0x7FFFFFFF : (int)tot;
+        }
+        __syncthreads();
+        const int total = s_total;
+        if (total == 0 || (sp.max_iters > 0 && it >= sp.max_iters)) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                ctrl->iters = it;
+                ctrl->done = total == 0 ? 1 : 0;
+                if (total != 0) ctrl->error = NP_ERR_NOT_CONVERGED;
+            }
+            return;
+        }
+        Cfg cfg = decide(sp, tb.cnt, it);
+        if (threadIdx.x == 0) build_tables(tb, pk_cfg(cfg));
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            ctrl->frontier_total += total;
+            record_trace(sp, it, cfg, tb.cnt, t_prev);
+        }
+        __syncthreads();
+        pk_expansion<UNIT>(st, q_cur, qm_cur, tb);
+        if (!grid_sync(ctrl, nb, gen)) return;
+    }
+}
+
+__global__ __launch_bounds__(WG) void k_pk_compact(DevState st, int it_off) {
+    Ctrl *ctrl = st.ctrl;
+    if (ld_coherent(&ctrl->done) || ld_coherent(&ctrl->error)) return;
+    const int it = ld_coherent(&ctrl->it_base) + it_off;
+    if (blockIdx.x == 0 && threadIdx.x < NCLS) st.cnt[((it + 2) % 3) * NCLS + threadIdx.x] = 0;
+    pk_compaction(st, st.cnt + (it % 3) * NCLS, st.queue[it & 1], st.qmask[it & 1]);
+}
+
+template <bool UNIT>
+__global__ __launch_bounds__(WG) void k_pk_expand(DevState st, int it_off) {
+    __shared__ ItemTables tb;
+    Ctrl *ctrl = st.ctrl;
+    if (ld_coherent(&ctrl->done) || ld_coherent(&ctrl->error) || ld_coherent(&ctrl->skip)) return;
+    const int it = ld_coherent(&ctrl->it_base) + it_off;
+    if (threadIdx.x < NCLS) {
+        tb.cnt[threadIdx.x] = ld_coherent(st.cnt + (it % 3) * NCLS + threadIdx.x);
+        tb.seg[threadIdx.x] = st.seg_off[threadIdx.x];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        Cfg cfg = make_cfg(ld_coherent(&ctrl->cfg_level), ld_coherent(&ctrl->cfg_klo),
+                           ld_coherent(&ctrl->cfg_khi));
+        build_tables(tb, pk_cfg(cfg));
+    }
+    __syncthreads();
+    pk_expansion<UNIT>(st, st.queue[it & 1], st.qmask[it & 1], tb);
+}
+
+__global__ void k_pk_init_sources(DevState st, const int *srcs, int nsrc) {
+    if (blockIdx.x != 0) return;
+    if (threadIdx.x == 0) {
+        st.ctrl->thresh = NO_WINDOW;
+        for (int k = 0; k < 3; ++k) st.ctrl->defer_min[k] = NO_WINDOW;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nsrc; i += blockDim.x) {
+        const int v = srcs[i];
+        st.dist[((long long)v << st.kshift) + i] = 0;
+        if (st.cls[v] == 0) continue;
+        atomicOr(&st.mflag[v], 1u << i);
+        atomicOr(&st.bm[0][v >> 5], 1u << (v & 31));
+    }
+}
+
+__global__ void k_unpack(const int *dist, int kshift, int lane, const int *old2new, int n, int *out) {
+    for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += (long long)gridDim.x * blockDim.x) {
+        const long long r = old2new ? old2new[v] : v;
+        out[v] = dist[(r << kshift) + lane];
+    }
+}
+
+// ------------------------------------------------------------------------------------
 // setup kernels
 // ------------------------------------------------------------------------------------
 __global__ void k_fill_i32(int *p, long long n, int v) {
@@ -963,11 +1292,14 @@ static const void *pick_persistent(bool batch, bool unit, bool push) {
     return push ? persistent_fn<false, false, true>() : persistent_fn<false, false, false>();
 }
 
-int persistent_max_wgs_per_cu(bool batch, bool unit, bool push) {
+static const void *pick_packed(bool unit) {
+    return unit ? (const void *)&k_pk_persistent<true> : (const void *)&k_pk_persistent<false>;
+}
+
+int persistent_max_wgs_per_cu(bool batch, bool unit, bool push, bool packed) {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pick_persistent(batch, unit, push), WG, 0) !=
-        hipSuccess)
-        return 0;
+    const void *fn = packed ? pick_packed(unit) : pick_persistent(batch, unit, push);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, WG, 0) != hipSuccess) return 0;
     return nb;
 }
 
@@ -976,15 +1308,34 @@ hipError_t launch_persistent(const DevState &st, const SchedDev &sp, bool batch,
     DevState a0 = st;
     SchedDev a1 = sp;
     void *args[] = {&a0, &a1};
-    return hipLaunchCooperativeKernel(pick_persistent(batch, unit, sp.push != 0), dim3(wgs), dim3(WG),
-                                      args, 0, s);
+    const void *fn = st.kshift >= 0 ? pick_packed(unit) : pick_persistent(batch, unit, sp.push != 0);
+    return hipLaunchCooperativeKernel(fn, dim3(wgs), dim3(WG), args, 0, s);
 }
 
 hipError_t launch_compact(const DevState &st, int it_off, bool batch, int wgs, hipStream_t s) {
     (void)batch;
     long long need = (st.nwords + WG - 1) / WG;
     int g = (int)(need < wgs ? (need < 1 ? 1 : need) : wgs);
-    hipLaunchKernelGGL(k_compact, dim3(g), dim3(WG), 0, s, st, it_off);
+    if (st.kshift >= 0) hipLaunchKernelGGL(k_pk_compact, dim3(g), dim3(WG), 0, s, st, it_off);
+    else hipLaunchKernelGGL(k_compact, dim3(g), dim3(WG), 0, s, st, it_off);
+    return hipGetLastError();
+}
+hipError_t launch_init_packed(const DevState &st, const int *d_srcs, int nsrc, hipStream_t s) {
+    if (st.kshift < 0 || nsrc < 1 || nsrc > (1 << st.kshift) || nsrc > PACK_MAX) return hipErrorInvalidValue;
+    hipError_t e;
+    if ((e = launch_fill_i32(st.dist, st.nslots << st.kshift, st.inf, s)) != hipSuccess) return e;
+    if ((e = launch_fill_u32(st.bm[0], st.nwords, 0u, s)) != hipSuccess) return e;
+    if ((e = launch_fill_u32(st.mflag, st.nslots, 0u, s)) != hipSuccess) return e;
+    if ((e = launch_fill_i32(st.cnt, 3 * NCLS, 0, s)) != hipSuccess) return e;
+    if ((e = launch_fill_i32((int *)st.ctrl, (long long)(sizeof(Ctrl) / sizeof(int)), 0, s)) != hipSuccess)
+        return e;
+    hipLaunchKernelGGL(k_pk_init_sources, dim3(1), dim3(64), 0, s, st, d_srcs, nsrc);
+    return hipGetLastError();
+}
+hipError_t launch_unpack(const int *dist, int kshift, int lane, const int *old2new, int n, int *out,
+                         hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_unpack, dim3(grid_for(n, 256)), dim3(256), 0, s, dist, kshift, lane, old2new, n, out);
     return hipGetLastError();
 }
 hipError_t launch_decide(const DevState &st, const SchedDev &sp, int it_off, hipStream_t s) {
@@ -994,6 +1345,11 @@ hipError_t launch_decide(const DevState &st, const SchedDev &sp, int it_off, hip
 hipError_t launch_expand(const DevState &st, const SchedDev &sp, int it_off, bool batch, bool unit,
                          int wgs, hipStream_t s) {
     const bool push = sp.push != 0;
+    if (st.kshift >= 0) {
+        if (unit) hipLaunchKernelGGL(k_pk_expand<true>, dim3(wgs), dim3(WG), 0, s, st, it_off);
+        else hipLaunchKernelGGL(k_pk_expand<false>, dim3(wgs), dim3(WG), 0, s, st, it_off);
+        return hipGetLastError();
+    }
 #define NP_EXP(B, U, P) hipLaunchKernelGGL((k_expand<B, U, P>), dim3(wgs), dim3(WG), 0, s, st, it_off)
     if (batch) {
         if (unit) { if (push) NP_EXP(true, true, true); else NP_EXP(true, true, false); }

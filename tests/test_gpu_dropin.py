@@ -66,3 +66,21 @@ def test_reader_failure_exits_like_reference(tmp_path):
     r = _run("gpu-sssp", ["-f", "0", "-i", str(bad), "-s", "6"], tmp_path)
     assert r.returncode == 0            # EXIT macro of common/global.h exits with status 0
     assert "info:" in r.stderr and "header says" in r.stderr
+
+
+def test_cli_beside_a_process_holding_the_cooperative_queue(tmp_path, npa, ctx, orc):
+    """This process runs persistent (cooperative) solves, then the CLI process solves on the
+    same device: it gets exact results whichever drive it is given."""
+    g = load_golden("d9_rand.npz")
+    G = npa.Graph.from_csr(ctx, g["row"], g["col"], g["w"])
+    src = int(g["source"])
+    d, st = ctx.sssp(G, src, npa.Sched(level=1))
+    k = list(g["sources"]).index(src)
+    assert np.array_equal(d, g["dist"][k]) and st.drive == npa.DRIVE_PERSISTENT
+    out = tmp_path / "costs.txt"
+    r = _run("gpu-sssp", ["-f", "0", "-i", os.path.join(GOLDEN, str(g["path"])), "-s", "6",
+                          "-o", str(out)], tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = out.read_text().splitlines()
+    assert lines == ["From Node %d -> %d is %d" % (src + 1, i + 1, x) for i, x in enumerate(g["dist"][k])]
+    G.free()

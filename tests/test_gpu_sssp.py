@@ -314,3 +314,60 @@ def test_relabel_modes(npa, orc, mode):
         E.free()
     finally:
         c.close()
+
+
+def _dev_to_host(npa, ctx, ptr, n):
+    out = np.empty(n, np.int32)
+    npa._check(npa.load().np_dev_copy(ctx.h, npa._ptr(out), ctypes.c_void_p(ptr), out.nbytes, 1))
+    return out
+
+
+def test_packed_batch(npa, ctx, orc):
+    """Sources sharing a graph are solved source-packed (2..32 per distance row): every
+    instance bit-exact against the oracle, for packs split over 32, renumbered and plain
+    graphs, duplicate and degree-0 sources, the three drives and the consolidation levels."""
+    G = npa.Graph.rmat(ctx, 13, 16, wmin=1, wmax=255, seed=31)
+    rng = np.random.default_rng(31)
+    E = npa.Graph.from_csr(ctx, *random_csr(rng, 5000, 40000, 1, 60)[:3])
+    small = [npa.Graph.rmat(ctx, 9 + i, 16, wmin=1, wmax=255, seed=310 + i) for i in range(2)]
+    hg, he = G.download(), E.download()
+    assert G.relabeled[0] and not E.relabeled[0]
+    srcs_g = [int(s) for s in npa.pick_sources(hg[0], 38, 31)]
+    deg = np.diff(hg[0])
+    zero = np.flatnonzero(deg == 0)
+    srcs_g += [srcs_g[0], int(zero[0]) if len(zero) else srcs_g[1]]   # duplicate, degree 0
+    graphs = [G] * len(srcs_g) + [E, E, E] + small
+    srcs = srcs_g + [0, 17, 4999] + [int(npa.pick_sources(g.download()[0], 1, 7)[0]) for g in small]
+    hosts = {id(G): hg, id(E): he}
+    for g in small:
+        hosts[id(g)] = g.download()
+    refs = [orc.dijkstra(*hosts[id(g)], s)[0] for g, s in zip(graphs, srcs)]
+    B = npa.Batch(ctx, graphs, srcs)
+    assert B.layout() == (3, 2)          # 40 sources -> 2 packs of 20, 3 on E -> 1 pack
+    for drive in (0, 1, 2):
+        for lv, lo, hi in ((1, 6, 6), (0, 6, 6), (2, 2, 4), (1, 30, 30), (2, 0, 0)):
+            st = B.solve(_sched(npa, lv, lo, hi, drive, 0))
+            assert st.iterations > 0
+            for i in range(len(graphs)):
+                assert np.array_equal(B.get(i), refs[i]), (drive, lv, lo, hi, i)
+    for i in (0, 39, 41, len(graphs) - 1):
+        assert np.array_equal(_dev_to_host(npa, ctx, B.device_dist(i), graphs[i].n), refs[i]), i
+    B.free()
+    Bu = npa.Batch(ctx, graphs, srcs, unit=True)
+    Bu.solve(_sched(npa, 2, 6, 6))
+    for i, (g, s) in enumerate(zip(graphs, srcs)):
+        r, c, _ = hosts[id(g)]
+        assert np.array_equal(Bu.get(i), orc.bfs_levels(r, c, s)), i
+    Bu.free()
+    ctx.set_option(npa.CTX_PACK, 0)
+    try:
+        Bc = npa.Batch(ctx, graphs, srcs)
+        assert Bc.layout() == (0, len(graphs))
+        Bc.solve(_sched(npa, 1, 6, 6))
+        for i in range(len(graphs)):
+            assert np.array_equal(Bc.get(i), refs[i]), i
+        Bc.free()
+    finally:
+        ctx.set_option(npa.CTX_PACK, 1)
+    for g in small + [E, G]:
+        g.free()
