@@ -168,6 +168,12 @@ def main():
     ap.add_argument("--cpu-sources", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--modes", default=None)
+    ap.add_argument("--sched", action="append", default=[],
+                    help="MODE=level,log2_t_lo,log2_t_hi,wgs_per_cu: use this schedule for MODE (tuned or "
+                         "batched) instead of running the autotuner (profiling runs hold only timed launches)")
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="after the run, copy this many bytes device-to-device 3 times (a copy kernel of known "
+                         "size for scaling FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only; --gpus %d is not supported "
@@ -192,7 +198,14 @@ def main():
     modes = [m for m in (args.modes or default_modes).split(",") if m]
     scheds = {"plain": plain, "level0": npa.Sched(level=0), "level1": npa.Sched(level=1),
               "level2": npa.Sched(level=2)}
-    if "tuned" in modes or "concat" in modes:
+    fixed = {}
+    for spec in args.sched:
+        mode, vals = spec.split("=")
+        lv, lo, hi, wg = (int(x) for x in vals.split(","))
+        fixed[mode] = npa.Sched(level=lv, log2_t_lo=lo, log2_t_hi=hi, wgs_per_cu=wg)
+    if "tuned" in fixed:
+        scheds["tuned"] = fixed["tuned"]
+    elif "tuned" in modes or "concat" in modes:
         t = time.time()
         big = max(range(len(W.graphs)), key=lambda i: W.graphs[i].m)
         srcs = [s for gi, s in W.inst if gi == big][:2]
@@ -216,7 +229,9 @@ def main():
             B = npa.Batch(ctx, [W.graphs[gi] for gi, _ in W.inst], [s for _, s in W.inst])
             ctx.set_option(npa.CTX_PACK, 1)
             batches[mode] = B
-            if mode == "batched":
+            if mode == "batched" and "batched" in fixed:
+                scheds[mode] = fixed["batched"]
+            elif mode == "batched":
                 t = time.time()
                 bs, bms = B.autotune(npa.Sched(level=1))
                 scheds[mode] = bs.copy(trace=0)
@@ -320,6 +335,15 @@ def main():
         "speedup_batched_vs_plain": round(results["batched"]["gteps"] / base, 3) if base and "batched" in results else None,
     }
     print(json.dumps(line), flush=True)
+    if args.copy_calibration > 0:
+        n4 = args.copy_calibration // 4
+        a, b = npa.DeviceBuffer(ctx, n4), npa.DeviceBuffer(ctx, n4)
+        for _ in range(3):
+            npa._check(npa.load().np_dev_copy(ctx.h, npa.ctypes.c_void_p(b.ptr), npa.ctypes.c_void_p(a.ptr),
+                                              n4 * 4, 2))
+        log("copy calibration: 3 device-to-device copies of %d bytes" % (n4 * 4))
+        a.free()
+        b.free()
     for B in batches.values():
         B.free()
     dbuf.free()

@@ -371,3 +371,27 @@ def test_packed_batch(npa, ctx, orc):
         ctx.set_option(npa.CTX_PACK, 1)
     for g in small + [E, G]:
         g.free()
+
+
+def test_learned_scheduler_beats_fixed_heuristic(npa, ctx, orc):
+    """The trained MLP (sched_mlp.f32, trained on R-MAT s18-22 seeds 100-299) against the
+    reference's fixed heuristic (block level, THRESHOLD 64) on a held-out R-MAT s20 graph:
+    identical distances, and faster (median of 5 timings per source)."""
+    path = os.path.join(os.path.dirname(npa.__file__), "sched_mlp.f32")
+    ctx.model_set(np.fromfile(path, dtype=np.float32))
+    G = npa.Graph.rmat(ctx, 20, 16, wmin=1, wmax=255, seed=7)
+    row, col, w = G.download()
+    srcs = [int(s) for s in npa.pick_sources(row, 4, 7)]
+    plain, learned = npa.Sched(level=1), npa.Sched(level=npa.LEVEL_LEARNED)
+    d_dist = npa.DeviceBuffer(ctx, G.n)
+    t_plain = t_learned = 0.0
+    for s in srcs:
+        ref, _ = orc.dijkstra(row, col, w, s)
+        for sch in (plain, learned):
+            ctx.sssp_device(G, s, d_dist.ptr, sch)       # warm
+            assert np.array_equal(d_dist.to_host(), ref)
+        t_plain += float(np.median([ctx.sssp_device(G, s, d_dist.ptr, plain).kernel_ms for _ in range(5)]))
+        t_learned += float(np.median([ctx.sssp_device(G, s, d_dist.ptr, learned).kernel_ms for _ in range(5)]))
+    d_dist.free()
+    G.free()
+    assert t_plain / t_learned >= 1.1, (t_plain, t_learned)
