@@ -169,7 +169,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--modes", default=None)
     ap.add_argument("--sched", action="append", default=[],
-                    help="MODE=level,log2_t_lo,log2_t_hi,wgs_per_cu: use this schedule for MODE (tuned or "
+                    help="MODE=level,log2_t_lo,log2_t_hi,wgs_per_cu[,drive]: use this schedule for MODE (tuned or "
                          "batched) instead of running the autotuner (profiling runs hold only timed launches)")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="after the run, copy this many bytes device-to-device 3 times (a copy kernel of known "
@@ -201,8 +201,9 @@ def main():
     fixed = {}
     for spec in args.sched:
         mode, vals = spec.split("=")
-        lv, lo, hi, wg = (int(x) for x in vals.split(","))
-        fixed[mode] = npa.Sched(level=lv, log2_t_lo=lo, log2_t_hi=hi, wgs_per_cu=wg)
+        v = [int(x) for x in vals.split(",")]
+        fixed[mode] = npa.Sched(level=v[0], log2_t_lo=v[1], log2_t_hi=v[2], wgs_per_cu=v[3],
+                                drive=v[4] if len(v) > 4 else npa.DRIVE_PERSISTENT)
     if "tuned" in fixed:
         scheds["tuned"] = fixed["tuned"]
     elif "tuned" in modes or "concat" in modes:
@@ -241,7 +242,7 @@ def main():
 
             def step(B=B, sch=scheds[mode]):
                 st = B.solve(sch)
-                if st.drive != npa.DRIVE_PERSISTENT:
+                if st.drive != sch.drive:
                     log("note: the batch ran on drive %d (no cooperative queue for this process)" % st.drive)
                 return [st.kernel_ms]
         else:
@@ -251,7 +252,7 @@ def main():
                 out = []
                 for gi, s in W.inst:
                     st = ctx.sssp_device(W.graphs[gi], s, dbuf.ptr, sch)
-                    if st.drive != npa.DRIVE_PERSISTENT:
+                    if st.drive != sch.drive:
                         log("note: a solve ran on drive %d (no cooperative queue for this process)" % st.drive)
                     out.append(st.kernel_ms)
                 return out
@@ -289,13 +290,21 @@ def main():
 
     head = max(results, key=lambda m: results[m]["gteps"])
     hr = results[head]
+    # A "launch" is one solve: one cooperative kernel on the persistent drive, or the
+    # compaction / decision / expansion kernels of every iteration on the graph and
+    # host-loop drives; its duration is the solve's device time (hipEvents on the
+    # library's stream).
+    persistent = scheds[head].drive == npa.DRIVE_PERSISTENT
     if head in batches:
         npk, ncat = batches[head].layout()
-        kernel = "k_pk_persistent" if npk and not ncat else ("k_persistent" if not npk else "k_pk_persistent+k_persistent")
+        names = ([("k_pk_persistent" if persistent else "k_pk_compact+k_decide+k_pk_expand")] if npk else []) + \
+                ([("k_persistent" if persistent else "k_compact+k_decide+k_expand")] if ncat else [])
+        kernel = " and ".join(names)
         per_launch_bytes = float(b_alg.sum()) / max(1, hr["launches_per_step"])
     else:
-        kernel = "k_persistent"
+        kernel = "k_persistent" if persistent else "k_compact+k_decide+k_expand"
         per_launch_bytes = float(b_alg.mean())
+    kernel += " (%s drive)" % {0: "persistent", 1: "host-loop", 2: "graph"}[scheds[head].drive]
     achieved = per_launch_bytes / (hr["launch_ms"] * 1e-3) / 1e9
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_%s.json" % args.config)

@@ -112,7 +112,9 @@ typedef struct np_stats {
     int32_t drive;          /* NP_DRIVE_* that ran: a persistent solve runs host-loop when the
                                process gets no cooperative queue (one holder per device) */
     int32_t pad;
-    int64_t reserved[5];
+    int64_t pairs_total;    /* sum over iterations of queued (vertex, source) pairs: equals
+                               frontier_total except in source-packed batch parts */
+    int64_t reserved[4];
 } np_stats;
 
 typedef struct np_ctx np_ctx;
@@ -296,9 +298,10 @@ int np_read_graph_stream(int32_t fmt, void *fp, int32_t *n, int64_t *m, int32_t 
                          int32_t **row, int32_t **col, int32_t **w);
 
 /* ---- autotuner -- no counterpart in the reference ----------------------------------
- * Times the candidate schedules (level x log2_t_lo x log2_t_hi x wgs_per_cu) on `g`
- * from the given sources and returns the fastest in *best (plain drive/frontier kept
- * from `base`).  ms_out (optional, cap entries) receives each candidate's mean time. */
+ * Times the candidate schedules (drive persistent or graph x level x log2_t_lo x
+ * log2_t_hi x wgs_per_cu) on `g` from the given sources and returns the fastest in *best
+ * (frontier mode and the other fields kept from `base`); *best_ms (optional) = its mean
+ * device time per source. */
 int np_autotune(np_ctx *ctx, np_graph *g, const int32_t *srcs, int32_t nsrc, const np_sched *base,
                 np_sched *best, double *best_ms);
 /* The same search on a batch: each candidate times one np_batch_solve (all parts);

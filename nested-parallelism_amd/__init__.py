@@ -67,7 +67,7 @@ class Sched(ctypes.Structure):
 class Stats(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int32), ("status", ctypes.c_int32), ("kernel_ms", ctypes.c_double),
                 ("frontier_total", ctypes.c_int64), ("drive", ctypes.c_int32), ("pad", ctypes.c_int32),
-                ("reserved", ctypes.c_int64 * 5)]
+                ("pairs_total", ctypes.c_int64), ("reserved", ctypes.c_int64 * 4)]
 
     def __repr__(self):
         return "Stats(iterations=%d, kernel_ms=%.4f, frontier_total=%d)" % (

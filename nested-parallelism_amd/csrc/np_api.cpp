@@ -376,6 +376,7 @@ static int run_drive(np_ctx *c, Space &s, const DevState &st, const np_sched &sc
         stats->status = rc;
         stats->kernel_ms = ms;
         stats->drive = drive;
+        stats->pairs_total = st.kshift >= 0 ? h.pairs_total : h.frontier_total;
         stats->frontier_total = h.frontier_total;
     }
     return rc;
@@ -908,6 +909,7 @@ int np_batch_solve(np_ctx *c, np_batch *b, const np_sched *sched, np_stats *stat
         acc.iterations += p.iterations;
         acc.kernel_ms += p.kernel_ms;
         acc.frontier_total += p.frontier_total;
+        acc.pairs_total += p.pairs_total;
         if (acc.drive < 0) acc.drive = p.drive;
     };
     // packs: scan frontier, no priority window
@@ -1247,13 +1249,16 @@ int np_autotune(np_ctx *c, np_graph *g, const int32_t *srcs, int32_t nsrc, const
     std::vector<int> scratch((size_t)g->n);
     int *d_dist = nullptr;
     HIPCHK(dalloc(&d_dist, (size_t)g->n));
+    for (int drive : {(int)NP_DRIVE_PERSISTENT, (int)NP_DRIVE_GRAPH})
     for (int lv = 0; lv <= 2; ++lv)
         for (int klo : klos)
             for (int khi : khis) {
                 if (lv == 0 && khi != khis[0]) continue;  // level 0 has no upper bin
                 if (khi < klo) continue;
                 for (int wg : wgs_opts) {
+                    if (drive != NP_DRIVE_PERSISTENT && wg != 0) continue;  // one grid size per kernel
                     np_sched s = b0;
+                    s.drive = drive;
                     s.level = lv; s.log2_t_lo = klo; s.log2_t_hi = khi; s.wgs_per_cu = wg;
                     s.trace = 0;
                     double tot = 0;
@@ -1285,13 +1290,16 @@ int np_batch_autotune(np_ctx *c, np_batch *b, const np_sched *base, np_sched *be
     const int khis[] = {6, 8, 10, 12, 14, 30};
     double bestt = 1e30;
     np_sched bs = b0;
+    for (int drive : {(int)NP_DRIVE_PERSISTENT, (int)NP_DRIVE_GRAPH})
     for (int lv = 0; lv <= 2; ++lv)
         for (int klo : klos)
             for (int khi : khis) {
                 if (lv == 0 && khi != khis[0]) continue;  // level 0 has no upper bin
                 if (khi < klo) continue;
                 for (int wg : wgs_opts) {
+                    if (drive != NP_DRIVE_PERSISTENT && wg != 0) continue;  // one grid size per kernel
                     np_sched s = b0;
+                    s.drive = drive;
                     s.level = lv; s.log2_t_lo = klo; s.log2_t_hi = khi; s.wgs_per_cu = wg;
                     s.trace = 0;
                     np_stats st;

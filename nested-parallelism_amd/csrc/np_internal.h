@@ -26,6 +26,7 @@ struct Ctrl {
     int skip;                 // host-loop / graph drives: this slot only raised the window
     int defer_cnt[3];         // per compaction pass (rotating): flagged vertices kept back
     int defer_min[3];         // ... and the least distance among them
+    long long pairs_total;    // source-packed: sum over iterations of queued (vertex, source) pairs
 };
 
 // Everything an expansion / compaction touches.  Indices are "slots": the vertex ids of

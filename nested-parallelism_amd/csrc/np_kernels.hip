@@ -845,6 +845,7 @@ __device__ void pk_expansion(const DevState &st, const int *q, const unsigned *q
 __device__ __forceinline__ void pk_compaction(const DevState &st, int *cnt_out, int *q_out, unsigned *qm_out) {
     __shared__ int s_lc[NCLS], s_lb[NCLS];
     unsigned *bm = st.bm[0];
+    int pairs = 0;
     for (long long wb = (long long)blockIdx.x * WG; wb < st.nwords; wb += (long long)gridDim.x * WG) {
         if (threadIdx.x < NCLS) s_lc[threadIdx.x] = 0;
         __syncthreads();
@@ -868,13 +869,18 @@ __device__ __forceinline__ void pk_compaction(const DevState &st, int *cnt_out, 
             if (!c) continue;
             const int p = atomicAdd(&s_lc[c], 1);
             const long long qi = st.seg_off[c] + s_lb[c] + p;
+            const unsigned msk = ld_coherent_u(st.mflag + v);
+            pairs += __popc(msk);
             st_coherent(q_out + qi, (int)v);
-            st_coherent_u(qm_out + qi, ld_coherent_u(st.mflag + v));
+            st_coherent_u(qm_out + qi, msk);
             st_coherent_u(st.mflag + v, 0u);
         }
         if (word) st_coherent_u(bm + wi, 0u);
         __syncthreads();
     }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) pairs += __shfl_xor(pairs, o);
+    if (lane_id() == 0 && pairs) atomicAdd((unsigned long long *)&st.ctrl->pairs_total, (unsigned long long)pairs);
 }
 
 // Lane-bin bound of a packed expansion (see PACK_LANE_CLS); the other bins keep theirs.
