@@ -96,7 +96,7 @@ def build_workload(npa, ctx, cfg, args):
         g = npa.Graph.grid(ctx, 4096, 4096, None, 1, 1000, seed=5)
         row = g.download()[0]
         srcs = npa.pick_sources(row, 32, 5)
-        return Workload(cfg, "SSSP, 4096x4096 grid, 4-neighbour bidirectional + 1%% random long-range edges (seed 5), "
+        return Workload(cfg, "SSSP, 4096x4096 grid, 4-neighbour bidirectional + 1% random long-range edges (seed 5), "
                              "weights int32 U[1,1000], 32 sources (seed 5)", [g], [(0, int(s)) for s in srcs], 1)
     raise SystemExit("unknown config %s" % cfg)
 
