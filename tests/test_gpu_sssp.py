@@ -395,3 +395,23 @@ def test_learned_scheduler_beats_fixed_heuristic(npa, ctx, orc):
     d_dist.free()
     G.free()
     assert t_plain / t_learned >= 1.1, (t_plain, t_learned)
+
+
+def test_learned_scheduler_not_worse_on_grids(npa, ctx, orc):
+    """Held-out grid (the trained network saw grids of other seeds): exact, and no slower
+    than the fixed heuristic beyond timing noise."""
+    path = os.path.join(os.path.dirname(npa.__file__), "sched_mlp.f32")
+    ctx.model_set(np.fromfile(path, dtype=np.float32))
+    G = npa.Graph.grid(ctx, 512, 512, None, 1, 1000, seed=7)
+    row, col, w = G.download()
+    s = int(npa.pick_sources(row, 1, 7)[0])
+    ref, _ = orc.dijkstra(row, col, w, s)
+    d_dist = npa.DeviceBuffer(ctx, G.n)
+    times = {}
+    for name, sch in (("plain", npa.Sched(level=1)), ("learned", npa.Sched(level=npa.LEVEL_LEARNED))):
+        ctx.sssp_device(G, s, d_dist.ptr, sch)
+        assert np.array_equal(d_dist.to_host(), ref), name
+        times[name] = float(np.median([ctx.sssp_device(G, s, d_dist.ptr, sch).kernel_ms for _ in range(5)]))
+    d_dist.free()
+    G.free()
+    assert times["learned"] <= 1.15 * times["plain"], times
