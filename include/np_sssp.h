@@ -86,7 +86,7 @@ typedef struct np_sched {
     int32_t graph_chunk; /* NP_DRIVE_GRAPH: iterations per captured chunk (0 = 16) */
     int32_t max_iters;   /* 0 = no cap beyond n+1 */
     int32_t trace;       /* nonzero: record per-iteration trace (np_trace_*) */
-    int32_t options;     /* NP_OPT_* bits */
+    int32_t options;     /* reserved, 0 */
     int32_t delta;       /* scan frontier: priority window.  0 = every flagged vertex is
                             expanded (the reference's unordered frontier); > 0 = only
                             vertices with dist < T are expanded, T rising to
@@ -95,10 +95,6 @@ typedef struct np_sched {
     int32_t reserved[5];
 } np_sched;
 
-/* np_sched.options: the check "dist[v] > alt" before the atomicMin reads dist through
- * the caches (may see an older, larger value: only more atomics, never a wrong result)
- * instead of a device-coherent load. */
-#define NP_OPT_CACHED_FILTER 1
 
 /* Fills the plain fixed-threshold schedule of the reference at `level` (0,1,2), the
  * persistent drive and the scan frontier. */
@@ -194,8 +190,9 @@ int np_bfs(np_ctx *ctx, np_graph *g, int32_t src, const np_sched *sched, int32_t
  *                 source pending at its tail (NP_CTX_PACK, default on)
  *   concatenated  every other instance: the graphs laid end to end in one slot space
  *                 with an instance-tagged worklist.
- * Packs always use the scan frontier without a priority window (np_sched.frontier and
- * .delta apply to the concatenated part); every other field applies to both. */
+ * Packs always use the scan frontier (np_sched.frontier applies to the concatenated
+ * part); their priority window (delta) admits a (vertex, source) pair when the source's
+ * distance at the vertex is below the window.  Every other field applies to both. */
 int np_batch_create(np_ctx *ctx, int32_t count, np_graph *const *graphs, const int32_t *srcs,
                     int32_t unit, np_batch **out);
 /* stats: iterations, kernel_ms and frontier_total summed over the parts. */
@@ -299,8 +296,9 @@ int np_read_graph_stream(int32_t fmt, void *fp, int32_t *n, int64_t *m, int32_t 
 
 /* ---- autotuner -- no counterpart in the reference ----------------------------------
  * Times the candidate schedules (drive persistent or graph x level x log2_t_lo x
- * log2_t_hi x wgs_per_cu) on `g` from the given sources and returns the fastest in *best
- * (frontier mode and the other fields kept from `base`); *best_ms (optional) = its mean
+ * log2_t_hi x wgs_per_cu) on `g` from the given sources, then the best of them with a
+ * priority window of 4, 16 and 64 times the mean edge weight, and returns the fastest in
+ * *best (frontier mode and the other fields kept from `base`); *best_ms (optional) = its mean
  * device time per source. */
 int np_autotune(np_ctx *ctx, np_graph *g, const int32_t *srcs, int32_t nsrc, const np_sched *base,
                 np_sched *best, double *best_ms);

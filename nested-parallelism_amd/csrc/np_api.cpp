@@ -412,7 +412,6 @@ static int solve_single(np_ctx *c, np_graph *g, int src, const np_sched *sched, 
     int *dist = (d_dist_user && !g->rl.active) ? d_dist_user : s.dist;
     const int inf = unit ? UNDEFINED : INF;
     DevState st = make_state(c, s, g->srow, g->scol, unit ? nullptr : g->sw, dist, inf);
-    st.cached_filter = (sch.options & NP_OPT_CACHED_FILTER) ? 1 : 0;
     SchedDev sd = make_schedd(c, sch, g->n);
     HIPCHK(hipMemcpyAsync(s.src_slots, &ssrc, sizeof(int), hipMemcpyHostToDevice, c->stream));
     HIPCHK(launch_init_solve(st, s.src_slots, 1, sd.push, sd.delta, c->stream));
@@ -912,20 +911,25 @@ int np_batch_solve(np_ctx *c, np_batch *b, const np_sched *sched, np_stats *stat
         acc.pairs_total += p.pairs_total;
         if (acc.drive < 0) acc.drive = p.drive;
     };
-    // packs: scan frontier, no priority window
+    // packs: scan frontier
     np_sched psch = sch;
     psch.frontier = NP_FRONTIER_SCAN;
-    psch.delta = 0;
     for (Pack *pk : b->packs) {
         np_graph *g = pk->g;
         Space &s = pk->sp;
         DevState st = make_state(c, s, g->srow, g->scol, unit ? nullptr : g->sw, s.dist, inf);
-        st.cached_filter = (sch.options & NP_OPT_CACHED_FILTER) ? 1 : 0;
         st.kshift = pk->kshift;
         st.mflag = s.mflag;
         st.qmask[0] = s.qmask[0]; st.qmask[1] = s.qmask[1];
         SchedDev sd = make_schedd(c, psch, g->n);
-        HIPCHK(launch_init_packed(st, s.src_slots, (int)pk->inst.size(), c->stream));
+        if (psch.max_iters <= 0) {
+            // Without a window every source's frontier empties within n+1 expansions; with
+            // one, every expansion takes the least pending (vertex, source) pair, whose
+            // distance is final, for the first time: at most n * sources expansions.
+            const long long cap = (long long)g->n * (long long)pk->inst.size() + 2;
+            sd.max_iters = cap > 0x7FFFFFFFll ? 0x7FFFFFFF : (int)cap;
+        }
+        HIPCHK(launch_init_packed(st, s.src_slots, (int)pk->inst.size(), sd.delta, c->stream));
         np_stats ps;
         if ((rc = run_drive(c, s, st, psch, sd, false, unit, &ps))) return rc;
         add(ps);
@@ -933,10 +937,13 @@ int np_batch_solve(np_ctx *c, np_batch *b, const np_sched *sched, np_stats *stat
     if (b->nconcat > 0) {
         Space &s = b->sp;
         DevState st = make_state(c, s, b->row, b->col, unit ? nullptr : b->w, s.dist, inf);
-        st.cached_filter = (sch.options & NP_OPT_CACHED_FILTER) ? 1 : 0;
         int nmax = 1;
-        for (int i = 0; i < b->count; ++i) if (b->pack_of[i] < 0) nmax = std::max(nmax, b->nverts[i]);
+        long long nsum = 2;
+        for (int i = 0; i < b->count; ++i)
+            if (b->pack_of[i] < 0) { nmax = std::max(nmax, b->nverts[i]); nsum += b->nverts[i]; }
         SchedDev sd = make_schedd(c, sch, nmax);
+        if (sch.max_iters <= 0 && sd.delta > 0)  // a window orders the instances' vertices together
+            sd.max_iters = nsum > 0x7FFFFFFFll ? 0x7FFFFFFF : (int)nsum;
         HIPCHK(launch_init_solve(st, s.src_slots, b->nconcat, sd.push, sd.delta, c->stream));
         np_stats ps;
         if ((rc = run_drive(c, s, st, sch, sd, true, unit, &ps))) return rc;
@@ -1232,6 +1239,33 @@ int np_step_end(np_ctx *c, int32_t *dist_out) {
 }
 
 // ---- autotuner ---------------------------------------------------------------------
+// Mean of up to 64K weights of g (the first ones of the solver layout): the scale of the
+// priority-window candidates.
+static int sample_mean_weight(np_ctx *c, const np_graph *g, double *wmean) {
+    *wmean = 0.0;
+    const long long k = g->m < 65536 ? g->m : 65536;
+    if (k <= 0 || !g->sw) return NP_OK;
+    std::vector<int> h((size_t)k);
+    HIPCHK(hipMemcpyAsync(h.data(), g->sw, sizeof(int) * (size_t)k, hipMemcpyDeviceToHost, c->stream));
+    int rc = ctx_sync(c);
+    if (rc) return rc;
+    double sum = 0;
+    for (int x : h) sum += x;
+    *wmean = sum / (double)k;
+    return NP_OK;
+}
+
+// Priority windows tried on top of the best schedule of the grid search.
+static std::vector<int> window_candidates(double wmean) {
+    std::vector<int> out;
+    if (wmean <= 0) return out;
+    for (double f : {4.0, 16.0, 64.0}) {
+        const double d = f * wmean;
+        if (d >= 1 && d < 1e9) out.push_back((int)d);
+    }
+    return out;
+}
+
 int np_autotune(np_ctx *c, np_graph *g, const int32_t *srcs, int32_t nsrc, const np_sched *base,
                 np_sched *best, double *best_ms) {
     if (!c || !g || !srcs || nsrc < 1 || !best) { set_error("bad arguments"); return NP_ERR_ARG; }
@@ -1271,6 +1305,22 @@ int np_autotune(np_ctx *c, np_graph *g, const int32_t *srcs, int32_t nsrc, const
                     if (tot < bestt) { bestt = tot; bs = s; }
                 }
             }
+    // priority window on top of the best scan-frontier schedule
+    double wmean = 0;
+    if (bs.frontier == NP_FRONTIER_SCAN && (rc = sample_mean_weight(c, g, &wmean)) != NP_OK) { dfree(d_dist); return rc; }
+    const np_sched grid_best = bs;
+    for (int delta : (bs.frontier == NP_FRONTIER_SCAN ? window_candidates(wmean) : std::vector<int>())) {
+        np_sched s = grid_best;
+        s.delta = delta;
+        double tot = 0;
+        for (int i = 0; i < nsrc && rc == NP_OK; ++i) {
+            np_stats st;
+            rc = solve_single(c, g, srcs[i], &s, d_dist, nullptr, false, &st);
+            tot += st.kernel_ms;
+        }
+        if (rc) { dfree(d_dist); return rc; }
+        if (tot < bestt) { bestt = tot; bs = s; }
+    }
     dfree(d_dist);
     *best = bs;
     if (best_ms) *best_ms = bestt / nsrc;
@@ -1307,6 +1357,19 @@ int np_batch_autotune(np_ctx *c, np_batch *b, const np_sched *base, np_sched *be
                     if (st.kernel_ms < bestt) { bestt = st.kernel_ms; bs = s; }
                 }
             }
+    // priority window on top of the best schedule, scaled by the weights of the graph
+    // that carries the most instances
+    double wmean = 0;
+    const np_graph *g0 = b->packs.empty() ? b->graphs[0] : b->packs[0]->g;
+    if ((rc = sample_mean_weight(c, g0, &wmean))) return rc;
+    const np_sched grid_best = bs;
+    for (int delta : (b->unit ? std::vector<int>() : window_candidates(wmean))) {
+        np_sched s = grid_best;
+        s.delta = delta;
+        np_stats st;
+        if ((rc = np_batch_solve(c, b, &s, &st))) return rc;
+        if (st.kernel_ms < bestt) { bestt = st.kernel_ms; bs = s; }
+    }
     *best = bs;
     if (best_ms) *best_ms = bestt;
     return NP_OK;

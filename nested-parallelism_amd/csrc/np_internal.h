@@ -46,7 +46,6 @@ struct DevState {
     long long nslots;
     long long nwords;         // bitmap words
     int inf;                  // unreached value written at init
-    int cached_filter;        // NP_OPT_CACHED_FILTER
     Ctrl *ctrl;
     // source-packed batch (kshift >= 0): 2^kshift sources of one graph per vertex row,
     // dist[(v << kshift) + i]; mflag[v] = sources pending at v; bm[0] = vertices with a
@@ -87,8 +86,9 @@ hipError_t launch_fill_lut(int *lut, long long first, long long count, int inst,
 hipError_t launch_copy_cls(const unsigned char *src, unsigned char *dst, long long n,
                            hipStream_t s);
 
-// Source-packed state: dist rows set to inf, source i of srcs[] at lane i (nsrc <= 2^kshift).
-hipError_t launch_init_packed(const DevState &st, const int *d_srcs, int nsrc, hipStream_t s);
+// Source-packed state: dist rows set to inf, source i of srcs[] at lane i (nsrc <= 2^kshift);
+// delta > 0 starts a priority window at T = delta.
+hipError_t launch_init_packed(const DevState &st, const int *d_srcs, int nsrc, int delta, hipStream_t s);
 // out[v] = dist[(map(v) << kshift) + lane], map = old2new or identity (old2new null).
 hipError_t launch_unpack(const int *dist, int kshift, int lane, const int *old2new, int n, int *out,
                          hipStream_t s);

@@ -179,6 +179,8 @@ __device__ __forceinline__ void relax_range(const DevState &st, const PushCtx &p
     for (int base = e0; base < e1; base += UNR * stride) {
         int tv[UNR], alt[UNR], cur[UNR];
         bool ok[UNR];
+        // every load is unconditional (lanes past the end read edge e0) so that all of them
+        // are in flight together; only the atomics are guarded
 #pragma unroll
         for (int k = 0; k < UNR; ++k) {
             int e = base + t + k * stride;
@@ -188,8 +190,7 @@ __device__ __forceinline__ void relax_range(const DevState &st, const PushCtx &p
             alt[k] = du + (UNIT ? 1 : st.w[ec]);
         }
 #pragma unroll
-        for (int k = 0; k < UNR; ++k)
-            cur[k] = ok[k] ? (st.cached_filter ? st.dist[tv[k]] : ld_coherent(st.dist + tv[k])) : 0;
+        for (int k = 0; k < UNR; ++k) cur[k] = ld_coherent(st.dist + tv[k]);
 #pragma unroll
         for (int k = 0; k < UNR; ++k) {
             bool want = false;
@@ -246,8 +247,7 @@ __device__ __forceinline__ void lane_tile(const DevState &st, const PushCtx &pc,
             alt[k] = ou + (UNIT ? 1 : st.w[ec]);
         }
 #pragma unroll
-        for (int k = 0; k < UNR; ++k)
-            cur[k] = ok[k] ? (st.cached_filter ? st.dist[tv[k]] : ld_coherent(st.dist + tv[k])) : 0;
+        for (int k = 0; k < UNR; ++k) cur[k] = ld_coherent(st.dist + tv[k]);
 #pragma unroll
         for (int k = 0; k < UNR; ++k) {
             bool want = false;
@@ -710,22 +710,19 @@ __device__ __forceinline__ void pk_relax_vertex(const DevState &st, int u, unsig
     while (e < e1) {
         int tv[UNR], alt[UNR], cur[UNR], src[UNR];
         bool ok[UNR];
+        // unconditional loads (lanes past the end read edge e0), guarded atomics
 #pragma unroll
         for (int k = 0; k < UNR; ++k) {
             ok[k] = e < e1;
             const int ec = ok[k] ? e : e0;
             src[k] = nth_set_bit(M, j);
             tv[k] = st.col[ec];
-            const int du = ld_coherent(du_row + src[k]);
-            alt[k] = ok[k] ? du + (UNIT ? 1 : st.w[ec]) : 0;
+            alt[k] = ld_coherent(du_row + src[k]) + (UNIT ? 1 : st.w[ec]);
             e += qs; j += rs;
             if (j >= P) { j -= P; e += 1; }
         }
 #pragma unroll
-        for (int k = 0; k < UNR; ++k) {
-            const int *p = st.dist + ((long long)tv[k] << ks) + src[k];
-            cur[k] = ok[k] ? (st.cached_filter ? *p : ld_coherent(p)) : 0;
-        }
+        for (int k = 0; k < UNR; ++k) cur[k] = ld_coherent(st.dist + ((long long)tv[k] << ks) + src[k]);
 #pragma unroll
         for (int k = 0; k < UNR; ++k)
             if (ok[k] && alt[k] < cur[k])
@@ -778,16 +775,14 @@ __device__ __forceinline__ void pk_lane_tile(const DevState &st, const int *q, c
             int jo = r - eo * Po;
             if (jo < 0) { eo -= 1; jo += Po; }
             else if (jo >= Po) { eo += 1; jo -= Po; }
+            // lanes past the end read the tile's first edge from vertex uo's row, source 0
             const int ec = ok[k] ? so + eo : e_safe;
             src[k] = ok[k] ? nth_set_bit(Mo, jo) : 0;
             tv[k] = st.col[ec];
-            alt[k] = ok[k] ? ld_coherent(st.dist + ((long long)uo << ks) + src[k]) + (UNIT ? 1 : st.w[ec]) : 0;
+            alt[k] = ld_coherent(st.dist + ((long long)uo << ks) + src[k]) + (UNIT ? 1 : st.w[ec]);
         }
 #pragma unroll
-        for (int k = 0; k < UNR; ++k) {
-            const int *p = st.dist + ((long long)tv[k] << ks) + src[k];
-            cur[k] = ok[k] ? (st.cached_filter ? *p : ld_coherent(p)) : 0;
-        }
+        for (int k = 0; k < UNR; ++k) cur[k] = ld_coherent(st.dist + ((long long)tv[k] << ks) + src[k]);
 #pragma unroll
         for (int k = 0; k < UNR; ++k)
             if (ok[k] && alt[k] < cur[k])
@@ -841,20 +836,53 @@ __device__ void pk_expansion(const DevState &st, const int *q, const unsigned *q
     }
 }
 
-// Vertices with pending sources -> class-segmented queue with their source masks.
-__device__ __forceinline__ void pk_compaction(const DevState &st, int *cnt_out, int *q_out, unsigned *qm_out) {
+// The pending sources of v (mask m) whose distance at v is below the window T.
+__device__ __forceinline__ unsigned pk_take(const DevState &st, long long v, unsigned m, int T) {
+    if (T == NO_WINDOW) return m;
+    const int *row = st.dist + (v << st.kshift);
+    unsigned take = 0;
+    for (unsigned b = m; b; b &= b - 1) {
+        const int i = __ffs(b) - 1;
+        if (ld_coherent(row + i) < T) take |= 1u << i;
+    }
+    return take;
+}
+
+// Vertices with pending sources -> class-segmented queue with their source masks.  With a
+// priority window (T < NO_WINDOW) only the (vertex, source) pairs with dist < T are queued;
+// the others stay pending, and the pass adds their count and least distance to
+// defer_cnt / defer_min.
+__device__ __forceinline__ void pk_compaction(const DevState &st, int *cnt_out, int *q_out, unsigned *qm_out,
+                                              int T, int *defer_cnt, int *defer_min) {
     __shared__ int s_lc[NCLS], s_lb[NCLS];
+    __shared__ int s_dc, s_dm;
     unsigned *bm = st.bm[0];
-    int pairs = 0;
+    int pairs = 0, my_dc = 0, my_dm = NO_WINDOW;
+    if (threadIdx.x == 0) { s_dc = 0; s_dm = NO_WINDOW; }
     for (long long wb = (long long)blockIdx.x * WG; wb < st.nwords; wb += (long long)gridDim.x * WG) {
         if (threadIdx.x < NCLS) s_lc[threadIdx.x] = 0;
         __syncthreads();
         const long long wi = wb + threadIdx.x;
         const unsigned word = wi < st.nwords ? ld_coherent_u(bm + wi) : 0u;
+        unsigned queued = 0, keep = 0;  // vertices of this word queued / still pending after the pass
         for (unsigned w = word; w; w &= w - 1) {
-            const long long v = wi * 32 + (__ffs(w) - 1);
+            const int b = __ffs(w) - 1;
+            const long long v = wi * 32 + b;
             const int c = st.cls[v];
-            if (c) atomicAdd(&s_lc[c], 1);  // class 0 is never flagged; kept out regardless
+            if (!c) continue;  // class 0 is never flagged; kept out regardless
+            const unsigned m = ld_coherent_u(st.mflag + v);
+            const unsigned t = pk_take(st, v, m, T);
+            if (t) { queued |= 1u << b; atomicAdd(&s_lc[c], 1); }
+            if (t != m) {
+                keep |= 1u << b;
+                const unsigned rest = m & ~t;
+                my_dc += __popc(rest);
+                const int *row = st.dist + (v << st.kshift);
+                for (unsigned r = rest; r; r &= r - 1) {
+                    const int d = ld_coherent(row + (__ffs(r) - 1));
+                    my_dm = d < my_dm ? d : my_dm;
+                }
+            }
         }
         __syncthreads();
         if (threadIdx.x < NCLS) {
@@ -863,24 +891,34 @@ __device__ __forceinline__ void pk_compaction(const DevState &st, int *cnt_out, 
             s_lc[threadIdx.x] = 0;
         }
         __syncthreads();
-        for (unsigned w = word; w; w &= w - 1) {
+        for (unsigned w = queued; w; w &= w - 1) {
             const long long v = wi * 32 + (__ffs(w) - 1);
             const int c = st.cls[v];
-            if (!c) continue;
             const int p = atomicAdd(&s_lc[c], 1);
             const long long qi = st.seg_off[c] + s_lb[c] + p;
-            const unsigned msk = ld_coherent_u(st.mflag + v);
-            pairs += __popc(msk);
+            const unsigned m = ld_coherent_u(st.mflag + v);
+            const unsigned t = pk_take(st, v, m, T);  // same answer as above: nothing writes dist now
+            pairs += __popc(t);
             st_coherent(q_out + qi, (int)v);
-            st_coherent_u(qm_out + qi, msk);
-            st_coherent_u(st.mflag + v, 0u);
+            st_coherent_u(qm_out + qi, t);
+            st_coherent_u(st.mflag + v, m & ~t);
         }
-        if (word) st_coherent_u(bm + wi, 0u);
+        if (word != keep) st_coherent_u(bm + wi, keep);
         __syncthreads();
     }
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) pairs += __shfl_xor(pairs, o);
+    for (int o = 32; o > 0; o >>= 1) {
+        pairs += __shfl_xor(pairs, o);
+        my_dc += __shfl_xor(my_dc, o);
+        const int x = __shfl_xor(my_dm, o);
+        my_dm = x < my_dm ? x : my_dm;
+    }
     if (lane_id() == 0 && pairs) atomicAdd((unsigned long long *)&st.ctrl->pairs_total, (unsigned long long)pairs);
+    if (T != NO_WINDOW) {
+        if (lane_id() == 0 && my_dc) { atomicAdd(&s_dc, my_dc); atomicMin(&s_dm, my_dm); }
+        __syncthreads();
+        if (threadIdx.x == 0 && s_dc) { atomicAdd(defer_cnt, s_dc); atomicMin(defer_min, s_dm); }
+    }
 }
 
 // Lane-bin bound of a packed expansion (see PACK_LANE_CLS); the other bins keep theirs.
@@ -897,6 +935,9 @@ __global__ __launch_bounds__(WG) void k_pk_persistent(DevState st, SchedDev sp) 
     unsigned long long gen = 0;
     const unsigned nb = gridDim.x;
     long long t_prev = 0;
+    // priority window: every workgroup computes the same sequence of thresholds
+    int T = sp.delta > 0 ? sp.delta : NO_WINDOW;
+    int pass = 0;
     for (int it = 0;; ++it) {
         int *cnt_cur = st.cnt + (it % 3) * NCLS;
         int *cnt_zero = st.cnt + ((it + 2) % 3) * NCLS;
@@ -908,19 +949,33 @@ __global__ __launch_bounds__(WG) void k_pk_persistent(DevState st, SchedDev sp) 
             t_prev = now;
         }
         if (blockIdx.x == 0 && threadIdx.x < NCLS) st_coherent(cnt_zero + threadIdx.x, 0);
-        pk_compaction(st, cnt_cur, q_cur, qm_cur);
-        if (!grid_sync(ctrl, nb, gen)) return;
-        if (threadIdx.x < NCLS) {
-            tb.cnt[threadIdx.x] = ld_coherent(cnt_cur + threadIdx.x);
-            tb.seg[threadIdx.x] = st.seg_off[threadIdx.x];
+        for (;;) {
+            int *dc = ctrl->defer_cnt + pass % 3, *dm = ctrl->defer_min + pass % 3;
+            // the statistics slot of the next pass was last read before the previous barrier
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                st_coherent(ctrl->defer_cnt + (pass + 1) % 3, 0);
+                st_coherent(ctrl->defer_min + (pass + 1) % 3, NO_WINDOW);
+            }
+            pk_compaction(st, cnt_cur, q_cur, qm_cur, T, dc, dm);
+            if (!grid_sync(ctrl, nb, gen)) return;
+            pass++;
+            if (threadIdx.x < NCLS) {
+                tb.cnt[threadIdx.x] = ld_coherent(cnt_cur + threadIdx.x);
+                tb.seg[threadIdx.x] = st.seg_off[threadIdx.x];
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                long long tot = 0;
+                for (int c = 1; c < NCLS; ++c) tot += tb.cnt[c];
+                s_total = tot > 0x7FFFFFFF ? 0x7FFFFFFF : (int)tot;
+            }
+            __syncthreads();
+            if (s_total > 0 || T == NO_WINDOW) break;
+            // nothing below the window: raise it to the least pending distance + delta
+            if (ld_coherent(dc) == 0) break;
+            const int mn = ld_coherent(dm);
+            T = mn > NO_WINDOW - sp.delta ? NO_WINDOW : mn + sp.delta;
         }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            long long tot = 0;
-            for (int c = 1; c < NCLS; ++c) tot += tb.cnt[c];
-            s_total = tot > 0x7FFFFFFF ? 0x7FFFFFFF : (int)tot;
-        }
-        __syncthreads();
         const int total = s_total;
         if (total == 0 || (sp.max_iters > 0 && it >= sp.max_iters)) {
             if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -946,8 +1001,15 @@ __global__ __launch_bounds__(WG) void k_pk_compact(DevState st, int it_off) {
     Ctrl *ctrl = st.ctrl;
     if (ld_coherent(&ctrl->done) || ld_coherent(&ctrl->error)) return;
     const int it = ld_coherent(&ctrl->it_base) + it_off;
+    const int pass = ld_coherent(&ctrl->pass);
+    const int T = ld_coherent(&ctrl->thresh);
     if (blockIdx.x == 0 && threadIdx.x < NCLS) st.cnt[((it + 2) % 3) * NCLS + threadIdx.x] = 0;
-    pk_compaction(st, st.cnt + (it % 3) * NCLS, st.queue[it & 1], st.qmask[it & 1]);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctrl->defer_cnt[(pass + 1) % 3] = 0;
+        ctrl->defer_min[(pass + 1) % 3] = NO_WINDOW;
+    }
+    pk_compaction(st, st.cnt + (it % 3) * NCLS, st.queue[it & 1], st.qmask[it & 1], T,
+                  ctrl->defer_cnt + pass % 3, ctrl->defer_min + pass % 3);
 }
 
 template <bool UNIT>
@@ -970,10 +1032,10 @@ __global__ __launch_bounds__(WG) void k_pk_expand(DevState st, int it_off) {
     pk_expansion<UNIT>(st, st.queue[it & 1], st.qmask[it & 1], tb);
 }
 
-__global__ void k_pk_init_sources(DevState st, const int *srcs, int nsrc) {
+__global__ void k_pk_init_sources(DevState st, const int *srcs, int nsrc, int T0) {
     if (blockIdx.x != 0) return;
     if (threadIdx.x == 0) {
-        st.ctrl->thresh = NO_WINDOW;
+        st.ctrl->thresh = T0;
         for (int k = 0; k < 3; ++k) st.ctrl->defer_min[k] = NO_WINDOW;
     }
     __syncthreads();
@@ -1326,7 +1388,7 @@ hipError_t launch_compact(const DevState &st, int it_off, bool batch, int wgs, h
     else hipLaunchKernelGGL(k_compact, dim3(g), dim3(WG), 0, s, st, it_off);
     return hipGetLastError();
 }
-hipError_t launch_init_packed(const DevState &st, const int *d_srcs, int nsrc, hipStream_t s) {
+hipError_t launch_init_packed(const DevState &st, const int *d_srcs, int nsrc, int delta, hipStream_t s) {
     if (st.kshift < 0 || nsrc < 1 || nsrc > (1 << st.kshift) || nsrc > PACK_MAX) return hipErrorInvalidValue;
     hipError_t e;
     if ((e = launch_fill_i32(st.dist, st.nslots << st.kshift, st.inf, s)) != hipSuccess) return e;
@@ -1335,7 +1397,7 @@ hipError_t launch_init_packed(const DevState &st, const int *d_srcs, int nsrc, h
     if ((e = launch_fill_i32(st.cnt, 3 * NCLS, 0, s)) != hipSuccess) return e;
     if ((e = launch_fill_i32((int *)st.ctrl, (long long)(sizeof(Ctrl) / sizeof(int)), 0, s)) != hipSuccess)
         return e;
-    hipLaunchKernelGGL(k_pk_init_sources, dim3(1), dim3(64), 0, s, st, d_srcs, nsrc);
+    hipLaunchKernelGGL(k_pk_init_sources, dim3(1), dim3(64), 0, s, st, d_srcs, nsrc, delta > 0 ? delta : NO_WINDOW);
     return hipGetLastError();
 }
 hipError_t launch_unpack(const int *dist, int kshift, int lane, const int *old2new, int n, int *out,

@@ -415,3 +415,25 @@ def test_learned_scheduler_not_worse_on_grids(npa, ctx, orc):
     d_dist.free()
     G.free()
     assert times["learned"] <= 1.15 * times["plain"], times
+
+
+@pytest.mark.parametrize("drive", [0, 1, 2])
+def test_packed_priority_window(npa, ctx, orc, drive):
+    """Source-packed batches with a priority window: only (vertex, source) pairs below the
+    window are expanded, the rest stay pending; distances unchanged."""
+    G = npa.Graph.grid(ctx, 160, 120, None, 1, 1000, seed=41)
+    R = npa.Graph.rmat(ctx, 12, 16, wmin=1, wmax=255, seed=41)
+    hg, hr = G.download(), R.download()
+    sg = [int(s) for s in npa.pick_sources(hg[0], 20, 41)]
+    sr = [int(s) for s in npa.pick_sources(hr[0], 5, 41)]
+    B = npa.Batch(ctx, [G] * len(sg) + [R] * len(sr), sg + sr)
+    assert B.layout() == (2, 0)
+    refs = [orc.dijkstra(*hg, s)[0] for s in sg] + [orc.dijkstra(*hr, s)[0] for s in sr]
+    for delta in (1, 300, 3000):
+        st = B.solve(_sched(npa, 1, 6, 6, drive, 0, delta=delta))
+        assert st.pairs_total > 0
+        for i, ref in enumerate(refs):
+            assert np.array_equal(B.get(i), ref), (delta, i)
+    B.free()
+    G.free()
+    R.free()
