@@ -221,10 +221,12 @@ __device__ __forceinline__ void lane_tile(const DevState &st, const PushCtx &pc,
     const int total = __shfl(incl, 63);
     const int excl = incl - d;
     if (total == 0) return;
-    // First edge of the tile (owner of index 0 = first lane with d > 0): a valid address
-    // for the lanes whose index runs past the end.
+    // First edge of the tile (owner of index 0 = first lane with d > 0) and that lane's slot
+    // base: the address the lanes whose index runs past the end read.  Edge and base come
+    // from the same lane, so the target slot lies inside that lane's instance.
     const int first_lane = __ffsll((long long)__ballot(d > 0)) - 1;
     const int e_safe = __shfl(s, first_lane);
+    const int b_safe = BATCH ? __shfl(B, first_lane) : 0;
     for (int base = 0; base < total; base += UNR * 64) {
         int tv[UNR], alt[UNR], cur[UNR];
         bool ok[UNR];
@@ -243,7 +245,7 @@ __device__ __forceinline__ void lane_tile(const DevState &st, const PushCtx &pc,
             const int ou = __shfl(du, o);
             const int ob = BATCH ? __shfl(B, o) : 0;
             const int ec = ok[k] ? e : e_safe;
-            tv[k] = ob + st.col[ec];
+            tv[k] = (ok[k] ? ob : b_safe) + st.col[ec];
             alt[k] = ou + (UNIT ? 1 : st.w[ec]);
         }
 #pragma unroll

@@ -437,3 +437,27 @@ def test_packed_priority_window(npa, ctx, orc, drive):
     B.free()
     G.free()
     R.free()
+
+
+@pytest.mark.parametrize("drive", [0, 2])
+def test_batch_large_instances_then_small_last(npa, ctx, orc, drive):
+    """Concatenated batch whose last instance is much smaller than the ones before it (the
+    shape of the uneven-graph batch): lane tiles that straddle instances address only slots
+    of their own instances."""
+    ctx.set_option(npa.CTX_PACK, 0)
+    try:
+        big = [npa.Graph.rmat(ctx, 15, 16, wmin=1, wmax=255, seed=50 + i) for i in range(3)]
+        tiny = npa.Graph.rmat(ctx, 6, 16, wmin=1, wmax=255, seed=59)
+        graphs = big + [tiny]
+        hosts = [g.download() for g in graphs]
+        srcs = [int(npa.pick_sources(h[0], 1, 50 + i)[0]) for i, h in enumerate(hosts)]
+        B = npa.Batch(ctx, graphs, srcs)
+        assert B.layout() == (0, 4)
+        B.solve(_sched(npa, 1, 6, 6, drive, 0))
+        for i, (h, s) in enumerate(zip(hosts, srcs)):
+            assert np.array_equal(B.get(i), orc.dijkstra(*h, s)[0]), i
+        B.free()
+        for g in graphs:
+            g.free()
+    finally:
+        ctx.set_option(npa.CTX_PACK, 1)
