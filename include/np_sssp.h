@@ -193,6 +193,7 @@ int np_bfs(np_ctx *ctx, np_graph *g, int32_t src, const np_sched *sched, int32_t
  * Packs always use the scan frontier (np_sched.frontier applies to the concatenated
  * part); their priority window (delta) admits a (vertex, source) pair when the source's
  * distance at the vertex is below the window.  Every other field applies to both. */
+/* The batch reads its graphs at every solve: free it before any graph it names. */
 int np_batch_create(np_ctx *ctx, int32_t count, np_graph *const *graphs, const int32_t *srcs,
                     int32_t unit, np_batch **out);
 /* stats: iterations, kernel_ms and frontier_total summed over the parts. */

@@ -400,6 +400,7 @@ static int solve_single(np_ctx *c, np_graph *g, int src, const np_sched *sched, 
                         int *h_dist, bool unit, np_stats *stats) {
     if (!c || !g) { set_error("null context or graph"); return NP_ERR_ARG; }
     if (g->ctx != c) { set_error("graph belongs to another context"); return NP_ERR_ARG; }
+    HIPCHK(hipSetDevice(c->device));
     if (src < 0 || src >= g->n) { set_error("source %d out of range [0,%d)", src, g->n); return NP_ERR_ARG; }
     np_sched sch;
     int rc = validate_sched(sched, &sch);
@@ -604,6 +605,7 @@ int np_dev_free(np_ctx *c, void *d_ptr) {
 int np_dev_copy(np_ctx *c, void *dst, const void *src, uint64_t bytes, int32_t kind) {
     if (!c || (!dst && bytes) || (!src && bytes)) { set_error("null argument"); return NP_ERR_ARG; }
     if (kind < 0 || kind > 2) { set_error("bad copy kind"); return NP_ERR_ARG; }
+    HIPCHK(hipSetDevice(c->device));
     if (!bytes) return NP_OK;
     const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
     HIPCHK(hipMemcpyAsync(dst, src, bytes, k, c->stream));
@@ -649,6 +651,7 @@ int np_graph_wrap_device(np_ctx *c, int32_t n, int64_t m, const int32_t *d_row, 
     int rc = new_graph(c, n, m, out);
     if (rc) return rc;
     if (!d_row || (m > 0 && (!d_col || !d_w))) { set_error("null CSR array"); return NP_ERR_ARG; }
+    HIPCHK(hipSetDevice(c->device));
     np_graph *g = new np_graph();
     g->ctx = c; g->n = n; g->m = m; g->owns = false;
     g->row = const_cast<int *>(d_row); g->col = const_cast<int *>(d_col); g->w = const_cast<int *>(d_w);
@@ -666,6 +669,7 @@ int np_graph_info(const np_graph *g, int32_t *n, int64_t *m) {
 
 int np_graph_download(np_ctx *c, const np_graph *g, int32_t *row, int32_t *col, int32_t *w) {
     if (!c || !g) { set_error("null argument"); return NP_ERR_ARG; }
+    HIPCHK(hipSetDevice(c->device));
     if (row) HIPCHK(hipMemcpyAsync(row, g->row, sizeof(int) * ((size_t)g->n + 1), hipMemcpyDeviceToHost, c->stream));
     if (col && g->m) HIPCHK(hipMemcpyAsync(col, g->col, sizeof(int) * (size_t)g->m, hipMemcpyDeviceToHost, c->stream));
     if (w && g->m) HIPCHK(hipMemcpyAsync(w, g->w, sizeof(int) * (size_t)g->m, hipMemcpyDeviceToHost, c->stream));
@@ -710,6 +714,7 @@ int np_bfs(np_ctx *c, np_graph *g, int32_t src, const np_sched *sched, int32_t *
 
 int np_reached_work(np_ctx *c, np_graph *g, const int32_t *d_dist, int32_t inf, int64_t *n_r, int64_t *m_r) {
     if (!c || !g || !d_dist) { set_error("null argument"); return NP_ERR_ARG; }
+    HIPCHK(hipSetDevice(c->device));
     unsigned long long h[2] = {0, 0};
     HIPCHK(launch_reached_work(g->row, d_dist, g->n, inf, c->d_work2, c->stream));
     HIPCHK(hipMemcpyAsync(h, c->d_work2, sizeof(h), hipMemcpyDeviceToHost, c->stream));
@@ -896,6 +901,7 @@ int np_batch_create(np_ctx *c, int32_t count, np_graph *const *graphs, const int
 
 int np_batch_solve(np_ctx *c, np_batch *b, const np_sched *sched, np_stats *stats) {
     if (!c || !b || b->ctx != c) { set_error("bad batch"); return NP_ERR_ARG; }
+    HIPCHK(hipSetDevice(c->device));
     np_sched sch;
     int rc = validate_sched(sched, &sch);
     if (rc) return rc;
@@ -977,6 +983,7 @@ static int batch_result(np_ctx *c, np_batch *b, int i, const int **res) {
 
 int np_batch_get(np_ctx *c, np_batch *b, int32_t i, int32_t *dist_out) {
     if (!c || !b || !dist_out || i < 0 || i >= b->count) { set_error("bad argument"); return NP_ERR_ARG; }
+    HIPCHK(hipSetDevice(c->device));
     const int *d = nullptr;
     int rc = batch_result(c, b, i, &d);
     if (rc) return rc;
@@ -1013,6 +1020,7 @@ int np_batch_free(np_batch *b) {
 // ---- learned scheduler -------------------------------------------------------------
 int np_model_set(np_ctx *c, const float *params) {
     if (!c || !params) { set_error("null argument"); return NP_ERR_ARG; }
+    HIPCHK(hipSetDevice(c->device));
     for (int i = 0; i < NP_MLP_NPARAM; ++i)
         if (!std::isfinite(params[i])) { set_error("parameter %d is not finite", i); return NP_ERR_ARG; }
     HIPCHK(hipMemcpyAsync(c->model, params, sizeof(float) * NP_MLP_NPARAM, hipMemcpyHostToDevice, c->stream));
@@ -1025,6 +1033,7 @@ int np_model_set(np_ctx *c, const float *params) {
 
 int np_model_get(np_ctx *c, float *params) {
     if (!c || !params) { set_error("null argument"); return NP_ERR_ARG; }
+    HIPCHK(hipSetDevice(c->device));
     if (!c->has_model) { set_error("no model set"); return NP_ERR_ARG; }
     HIPCHK(hipMemcpyAsync(params, c->model, sizeof(float) * NP_MLP_NPARAM, hipMemcpyDeviceToHost, c->stream));
     return ctx_sync(c);
@@ -1068,6 +1077,7 @@ int np_model_loss_grad(np_ctx *c, const float *x, const int32_t *label, const fl
                        float lam, float *loss, float *grad) {
     int rc = check_train_args(c, x, label, target, batch);
     if (rc) return rc;
+    HIPCHK(hipSetDevice(c->device));
     if ((rc = ensure_model(c))) return rc;
     TrainBufs tb;
     if ((rc = upload_train(c, tb, x, label, target, batch, 1))) return rc;
@@ -1082,6 +1092,7 @@ int np_model_train(np_ctx *c, const float *x, const int32_t *label, const float 
     int rc = check_train_args(c, x, label, target, batch);
     if (rc) return rc;
     if (epochs < 1) { set_error("epochs must be >= 1"); return NP_ERR_ARG; }
+    HIPCHK(hipSetDevice(c->device));
     if ((rc = ensure_model(c))) return rc;
     if (!c->adam_m) {
         HIPCHK(dalloc(&c->adam_m, NP_MLP_NPARAM));
@@ -1105,6 +1116,7 @@ int np_model_train(np_ctx *c, const float *x, const int32_t *label, const float 
 
 int np_model_forward(np_ctx *c, const float *x, int32_t batch, float *out) {
     if (!c || !x || !out || batch < 1) { set_error("bad arguments"); return NP_ERR_ARG; }
+    HIPCHK(hipSetDevice(c->device));
     int rc = ensure_model(c);
     if (rc) return rc;
     float *dx = nullptr, *dout = nullptr;
@@ -1141,6 +1153,7 @@ static int step_compact(np_ctx *c) {
 int np_step_begin(np_ctx *c, np_graph *g, int32_t src, np_stats *stats) {
     if (!c || !g || g->ctx != c) { set_error("bad arguments"); return NP_ERR_ARG; }
     if (src < 0 || src >= g->n) { set_error("source out of range"); return NP_ERR_ARG; }
+    HIPCHK(hipSetDevice(c->device));
     int rc = graph_space(c, g);
     if (rc) return rc;
     if (c->snap_n < g->n) {
@@ -1269,6 +1282,7 @@ static std::vector<int> window_candidates(double wmean) {
 int np_autotune(np_ctx *c, np_graph *g, const int32_t *srcs, int32_t nsrc, const np_sched *base,
                 np_sched *best, double *best_ms) {
     if (!c || !g || !srcs || nsrc < 1 || !best) { set_error("bad arguments"); return NP_ERR_ARG; }
+    HIPCHK(hipSetDevice(c->device));
     np_sched b0;
     int rc = validate_sched(base, &b0);
     if (rc) return rc;
@@ -1329,6 +1343,7 @@ int np_autotune(np_ctx *c, np_graph *g, const int32_t *srcs, int32_t nsrc, const
 
 int np_batch_autotune(np_ctx *c, np_batch *b, const np_sched *base, np_sched *best, double *best_ms) {
     if (!c || !b || b->ctx != c || !best) { set_error("bad arguments"); return NP_ERR_ARG; }
+    HIPCHK(hipSetDevice(c->device));
     np_sched b0;
     int rc = validate_sched(base, &b0);
     if (rc) return rc;
