@@ -190,13 +190,15 @@ int np_bfs(np_ctx *ctx, np_graph *g, int32_t src, const np_sched *sched, int32_t
  *                 source pending at its tail (NP_CTX_PACK, default on)
  *   concatenated  every other instance: the graphs laid end to end in one slot space
  *                 with an instance-tagged worklist.
- * Packs always use the scan frontier (np_sched.frontier applies to the concatenated
- * part); their priority window (delta) admits a (vertex, source) pair when the source's
+ * On the graph drive the parts run side by side, each on its own stream (without a trace);
+ * on the other drives one after another.  Packs always use the scan frontier
+ * (np_sched.frontier applies to the concatenated part); their priority window (delta) admits a (vertex, source) pair when the source's
  * distance at the vertex is below the window.  Every other field applies to both. */
 /* The batch reads its graphs at every solve: free it before any graph it names. */
 int np_batch_create(np_ctx *ctx, int32_t count, np_graph *const *graphs, const int32_t *srcs,
                     int32_t unit, np_batch **out);
-/* stats: iterations, kernel_ms and frontier_total summed over the parts. */
+/* stats: iterations and frontier_total summed over the parts; kernel_ms = device time of the
+ * whole solve. */
 int np_batch_solve(np_ctx *ctx, np_batch *b, const np_sched *sched, np_stats *stats);
 /* Host copy of instance i's distances (int32[n_i]). */
 int np_batch_get(np_ctx *ctx, np_batch *b, int32_t i, int32_t *dist_out);

@@ -75,6 +75,11 @@ struct Space {
     int src_cap = 0;
     unsigned *mflag = nullptr;    // source-packed: pending sources per vertex
     unsigned *qmask[2] = {nullptr, nullptr};
+    // a batch part run concurrently with the others: its own control block, stream, event
+    Ctrl *pctrl = nullptr;
+    Ctrl *h_pctrl = nullptr;
+    hipStream_t pstream = nullptr;
+    hipEvent_t pev = nullptr;
     // captured chunk of the graph drive
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
@@ -90,6 +95,11 @@ struct Space {
         cls = nullptr;
         dfree(lut); dfree(inst); dfree(src_slots);
         dfree(mflag); dfree(qmask[0]); dfree(qmask[1]);
+        dfree(pctrl);
+        if (h_pctrl) (void)hipHostFree(h_pctrl);
+        if (pstream) (void)hipStreamDestroy(pstream);
+        if (pev) (void)hipEventDestroy(pev);
+        h_pctrl = nullptr; pstream = nullptr; pev = nullptr;
         drop_graph();
     }
     void drop_graph() {
@@ -287,6 +297,42 @@ static int read_ctrl(np_ctx *c) {
     return ctx_sync(c);
 }
 
+// Captures (or reuses) the graph drive's chunk of sch.graph_chunk iterations for this state.
+static int ensure_graph(np_ctx *c, Space &s, const DevState &st, const np_sched &sch, const SchedDev &sd,
+                        bool batch, bool unit) {
+    const int K = sch.graph_chunk > 0 ? sch.graph_chunk : 16;
+    const int wgs = c->num_cus * 8;
+    bool same = s.gexec && s.gchunk == K && s.gwgs == wgs && s.gbatch == batch && s.gunit == unit &&
+                std::memcmp(&s.gsp, &sd, sizeof(SchedDev)) == 0 &&
+                std::memcmp(&s.gst, &st, sizeof(DevState)) == 0;
+    if (same) return NP_OK;
+    s.drop_graph();
+    hipStream_t cs = nullptr;
+    HIPCHK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        for (int j = 0; j < K && e == hipSuccess; ++j) {
+            if (!sd.push) e = launch_compact(st, j, batch, wgs, cs);
+            if (e == hipSuccess) e = launch_decide(st, sd, j, cs);
+            if (e == hipSuccess) e = launch_expand(st, sd, j, batch, unit, wgs, cs);
+        }
+        if (e == hipSuccess) e = launch_advance(st, K, cs);
+        hipGraph_t g = nullptr;
+        hipError_t e2 = hipStreamEndCapture(cs, &g);
+        if (e == hipSuccess) e = e2;
+        if (e == hipSuccess) {
+            s.graph = g;
+            e = hipGraphInstantiate(&s.gexec, g, nullptr, nullptr, 0);
+        } else if (g) {
+            (void)hipGraphDestroy(g);
+        }
+    }
+    (void)hipStreamDestroy(cs);
+    if (e != hipSuccess) { s.drop_graph(); return hip_fail(e, "graph capture", __LINE__); }
+    s.gchunk = K; s.gwgs = wgs; s.gbatch = batch; s.gunit = unit; s.gsp = sd; s.gst = st;
+    return NP_OK;
+}
+
 // Runs a solve whose state was initialised; fills stats.
 static int run_drive(np_ctx *c, Space &s, const DevState &st, const np_sched &sch, const SchedDev &sd,
                      bool batch, bool unit, np_stats *stats) {
@@ -323,37 +369,7 @@ static int run_drive(np_ctx *c, Space &s, const DevState &st, const np_sched &sc
         HIPCHK(hipEventRecord(c->ev1, c->stream));
         if ((rc = read_ctrl(c))) return rc;
     } else if (drive == NP_DRIVE_GRAPH) {
-        const int K = sch.graph_chunk > 0 ? sch.graph_chunk : 16;
-        const int wgs = c->num_cus * 8;
-        bool same = s.gexec && s.gchunk == K && s.gwgs == wgs && s.gbatch == batch && s.gunit == unit &&
-                    std::memcmp(&s.gsp, &sd, sizeof(SchedDev)) == 0 &&
-                    std::memcmp(&s.gst, &st, sizeof(DevState)) == 0;
-        if (!same) {
-            s.drop_graph();
-            hipStream_t cs = nullptr;
-            HIPCHK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
-            hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
-            if (e == hipSuccess) {
-                for (int j = 0; j < K && e == hipSuccess; ++j) {
-                    if (!sd.push) e = launch_compact(st, j, batch, wgs, cs);
-                    if (e == hipSuccess) e = launch_decide(st, sd, j, cs);
-                    if (e == hipSuccess) e = launch_expand(st, sd, j, batch, unit, wgs, cs);
-                }
-                if (e == hipSuccess) e = launch_advance(st, K, cs);
-                hipGraph_t g = nullptr;
-                hipError_t e2 = hipStreamEndCapture(cs, &g);
-                if (e == hipSuccess) e = e2;
-                if (e == hipSuccess) {
-                    s.graph = g;
-                    e = hipGraphInstantiate(&s.gexec, g, nullptr, nullptr, 0);
-                } else if (g) {
-                    (void)hipGraphDestroy(g);
-                }
-            }
-            (void)hipStreamDestroy(cs);
-            if (e != hipSuccess) { s.drop_graph(); return hip_fail(e, "graph capture", __LINE__); }
-            s.gchunk = K; s.gwgs = wgs; s.gbatch = batch; s.gunit = unit; s.gsp = sd; s.gst = st;
-        }
+        if ((rc = ensure_graph(c, s, st, sch, sd, batch, unit))) return rc;
         HIPCHK(hipEventRecord(c->ev0, c->stream));
         for (;;) {
             HIPCHK(hipGraphLaunch(s.gexec, c->stream));
@@ -899,6 +915,73 @@ int np_batch_create(np_ctx *c, int32_t count, np_graph *const *graphs, const int
     return NP_OK;
 }
 
+// One part of a batch solve.
+struct PartJob {
+    Space *s;
+    DevState st;
+    SchedDev sd;
+    bool batch;
+    bool done;
+};
+
+static int part_resources(Space &s) {
+    if (s.pstream) return NP_OK;
+    HIPCHK(dalloc(&s.pctrl, 1));
+    HIPCHK(hipHostMalloc((void **)&s.h_pctrl, sizeof(Ctrl), hipHostMallocDefault));
+    HIPCHK(hipStreamCreateWithFlags(&s.pstream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&s.pev, hipEventDisableTiming));
+    return NP_OK;
+}
+
+// Graph drive over several parts at once: each part replays its captured chunk on its own
+// stream with its own control block, so the parts' kernels overlap on the device; the host
+// polls every part after each round of chunks.  Device time is taken from a fork event on
+// the context stream to the join of every part stream.
+static int run_parts_concurrent(np_ctx *c, std::vector<PartJob> &jobs, const np_sched &sch, bool unit,
+                                np_stats *acc) {
+    int rc = NP_OK;
+    for (PartJob &j : jobs) {
+        if ((rc = ensure_graph(c, *j.s, j.st, sch, j.sd, j.batch, unit))) return rc;
+        HIPCHK(hipStreamSynchronize(j.s->pstream));  // initialisation done
+    }
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    for (PartJob &j : jobs) HIPCHK(hipStreamWaitEvent(j.s->pstream, c->ev0, 0));
+    size_t active = jobs.size();
+    while (active > 0) {
+        for (PartJob &j : jobs) {
+            if (j.done) continue;
+            HIPCHK(hipGraphLaunch(j.s->gexec, j.s->pstream));
+            HIPCHK(hipMemcpyAsync(j.s->h_pctrl, j.s->pctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, j.s->pstream));
+        }
+        for (PartJob &j : jobs) {
+            if (j.done) continue;
+            HIPCHK(hipStreamSynchronize(j.s->pstream));
+            const Ctrl &h = *j.s->h_pctrl;
+            if (h.done || h.error) { j.done = true; --active; }
+        }
+    }
+    for (PartJob &j : jobs) {
+        HIPCHK(hipEventRecord(j.s->pev, j.s->pstream));
+        HIPCHK(hipStreamWaitEvent(c->stream, j.s->pev, 0));
+    }
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    HIPCHK(hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    acc->kernel_ms = ms;
+    acc->drive = NP_DRIVE_GRAPH;
+    for (PartJob &j : jobs) {
+        const Ctrl &h = *j.s->h_pctrl;
+        acc->iterations += h.iters;
+        acc->frontier_total += h.frontier_total;
+        acc->pairs_total += j.st.kshift >= 0 ? h.pairs_total : h.frontier_total;
+        if (h.error && rc == NP_OK) rc = h.error;
+    }
+    if (rc == NP_ERR_NOT_CONVERGED) set_error("iteration cap reached");
+    if (rc == NP_ERR_TIMEOUT) set_error("a grid barrier timed out; the kernel exited early");
+    return rc;
+}
+
 int np_batch_solve(np_ctx *c, np_batch *b, const np_sched *sched, np_stats *stats) {
     if (!c || !b || b->ctx != c) { set_error("bad batch"); return NP_ERR_ARG; }
     HIPCHK(hipSetDevice(c->device));
@@ -920,6 +1003,10 @@ int np_batch_solve(np_ctx *c, np_batch *b, const np_sched *sched, np_stats *stat
     // packs: scan frontier
     np_sched psch = sch;
     psch.frontier = NP_FRONTIER_SCAN;
+    const int nparts = (int)b->packs.size() + (b->nconcat > 0 ? 1 : 0);
+    // the graph drive runs the parts side by side, each on its own stream
+    const bool together = sch.drive == NP_DRIVE_GRAPH && nparts > 1 && !sch.trace;
+    std::vector<PartJob> jobs;
     for (Pack *pk : b->packs) {
         np_graph *g = pk->g;
         Space &s = pk->sp;
@@ -934,6 +1021,13 @@ int np_batch_solve(np_ctx *c, np_batch *b, const np_sched *sched, np_stats *stat
             // distance is final, for the first time: at most n * sources expansions.
             const long long cap = (long long)g->n * (long long)pk->inst.size() + 2;
             sd.max_iters = cap > 0x7FFFFFFFll ? 0x7FFFFFFF : (int)cap;
+        }
+        if (together) {
+            if ((rc = part_resources(s))) return rc;
+            st.ctrl = s.pctrl;
+            HIPCHK(launch_init_packed(st, s.src_slots, (int)pk->inst.size(), sd.delta, s.pstream));
+            jobs.push_back(PartJob{&s, st, sd, false, false});
+            continue;
         }
         HIPCHK(launch_init_packed(st, s.src_slots, (int)pk->inst.size(), sd.delta, c->stream));
         np_stats ps;
@@ -950,10 +1044,23 @@ int np_batch_solve(np_ctx *c, np_batch *b, const np_sched *sched, np_stats *stat
         SchedDev sd = make_schedd(c, sch, nmax);
         if (sch.max_iters <= 0 && sd.delta > 0)  // a window orders the instances' vertices together
             sd.max_iters = nsum > 0x7FFFFFFFll ? 0x7FFFFFFF : (int)nsum;
-        HIPCHK(launch_init_solve(st, s.src_slots, b->nconcat, sd.push, sd.delta, c->stream));
-        np_stats ps;
-        if ((rc = run_drive(c, s, st, sch, sd, true, unit, &ps))) return rc;
-        add(ps);
+        if (together) {
+            if ((rc = part_resources(s))) return rc;
+            st.ctrl = s.pctrl;
+            HIPCHK(launch_init_solve(st, s.src_slots, b->nconcat, sd.push, sd.delta, s.pstream));
+            jobs.push_back(PartJob{&s, st, sd, true, false});
+        } else {
+            HIPCHK(launch_init_solve(st, s.src_slots, b->nconcat, sd.push, sd.delta, c->stream));
+            np_stats ps;
+            if ((rc = run_drive(c, s, st, sch, sd, true, unit, &ps))) return rc;
+            add(ps);
+        }
+    }
+    if (together) {
+        // packs and the concatenated part share one schedule apart from the frontier mode
+        rc = run_parts_concurrent(c, jobs, psch, unit, &acc);
+        if (stats) { *stats = acc; stats->status = rc; }
+        return rc;
     }
     if (stats) *stats = acc;
     return NP_OK;
