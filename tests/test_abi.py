@@ -63,3 +63,29 @@ def test_sched_defaults(npa):
     assert (s.level, s.log2_t_lo, s.log2_t_hi, s.drive, s.frontier) == (1, 6, 6, 0, 0)
     s = npa.default_sched(9)
     assert s.level == 1
+
+
+def test_struct_layouts_match_header(npa, tmp_path):
+    """np_sched, np_stats and np_trace_rec as the C compiler lays them out from the header
+    equal the ctypes mirror (size and every field offset)."""
+    structs = {"np_sched": npa.Sched, "np_stats": npa.Stats, "np_trace_rec": npa.TraceRec}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "np_sssp.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append('printf("%s size %%zu\\n", sizeof(%s));' % (cname, cname))
+        for f, _ in py._fields_:
+            lines.append('printf("%s %s %%zu\\n", offsetof(%s, %s));' % (cname, f, cname, f))
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], stdout=subprocess.PIPE, text=True, check=True).stdout.split("\n")
+    got = {}
+    for l in out:
+        if l.strip():
+            parts = l.split()
+            got[(parts[0], parts[1])] = int(parts[2])
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
