@@ -192,7 +192,7 @@ def main():
     nmax = max(g.n for g in W.graphs)
     dbuf = npa.DeviceBuffer(ctx, nmax)
 
-    default_modes = {"c1": "plain,tuned", "c2": "level0,level1,level2,tuned",
+    default_modes = {"c1": "plain,tuned", "c2": "level0,level1,level2,tuned,batched",
                      "c3": "plain,tuned,learned,concat,batched",
                      "c4": "plain,tuned,concat,batched", "c5": "plain,tuned,learned,batched"}[args.config]
     modes = [m for m in (args.modes or default_modes).split(",") if m]
