@@ -74,6 +74,7 @@ struct Space {
     int *src_slots = nullptr;
     int src_cap = 0;
     unsigned *mflag = nullptr;    // source-packed: pending sources per vertex
+    unsigned *mtake = nullptr;    // source-packed: compaction scratch per vertex
     unsigned *qmask[2] = {nullptr, nullptr};
     // a batch part run concurrently with the others: its own control block, stream, event
     Ctrl *pctrl = nullptr;
@@ -94,7 +95,7 @@ struct Space {
         if (own_cls) dfree(cls);
         cls = nullptr;
         dfree(lut); dfree(inst); dfree(src_slots);
-        dfree(mflag); dfree(qmask[0]); dfree(qmask[1]);
+        dfree(mflag); dfree(mtake); dfree(qmask[0]); dfree(qmask[1]);
         dfree(pctrl);
         if (h_pctrl) (void)hipHostFree(h_pctrl);
         if (pstream) (void)hipStreamDestroy(pstream);
@@ -778,6 +779,7 @@ static int build_pack(np_ctx *c, np_batch *b, np_graph *g, const std::vector<int
     if (rc) return rc;
     HIPCHK(dalloc(&s.dist, (size_t)g->n << ks));
     HIPCHK(dalloc(&s.mflag, (size_t)g->n));
+    HIPCHK(dalloc(&s.mtake, (size_t)g->n));
     HIPCHK(dalloc(&s.qmask[0], (size_t)s.qcap));
     HIPCHK(dalloc(&s.qmask[1], (size_t)s.qcap));
     if ((rc = ensure_src(c, s, k))) return rc;
@@ -1013,6 +1015,7 @@ int np_batch_solve(np_ctx *c, np_batch *b, const np_sched *sched, np_stats *stat
         DevState st = make_state(c, s, g->srow, g->scol, unit ? nullptr : g->sw, s.dist, inf);
         st.kshift = pk->kshift;
         st.mflag = s.mflag;
+        st.mtake = s.mtake;
         st.qmask[0] = s.qmask[0]; st.qmask[1] = s.qmask[1];
         SchedDev sd = make_schedd(c, psch, g->n);
         if (psch.max_iters <= 0) {

@@ -18,7 +18,8 @@
 // A vertex whose distance improved is flagged in an update bitmap (the reference's
 // update[] byte array, one bit per vertex) and, for the next iteration, either compacted
 // from the bitmap (scan frontier, unorder_generateQueue_kernel :648-658) or appended by
-// the relaxing wavefront (push frontier).
+// the relaxing wavefront (push frontier).  Batches of up to 32 sources of one graph run
+// the same bins over (vertex, pending sources) work items ("source-packed batch" below).
 #include <hip/hip_runtime.h>
 #include "np_internal.h"
 
@@ -171,16 +172,19 @@ __device__ __forceinline__ void push_wave(const DevState &st, const PushCtx &pc,
 
 constexpr int UNR = 4;
 
+// GUARD: lanes past the end of their range issue no loads (per-iteration kernels, eight
+// waves per SIMD: enough loads in flight already); otherwise every load is issued and only
+// the atomics are guarded, so the unrolled items' loads are all in flight together
+// (persistent kernels, four to five waves per SIMD).  Dead lanes then read a valid address.
+
 // Edges [e0, e1) of one vertex (slot base B, distance du), worker t of `stride`.
 // The loop trip count is uniform across the wavefront.
-template <bool UNIT, bool PUSH>
+template <bool UNIT, bool PUSH, bool GUARD>
 __device__ __forceinline__ void relax_range(const DevState &st, const PushCtx &pc, int B, int e0,
                                             int e1, int du, int t, int stride) {
     for (int base = e0; base < e1; base += UNR * stride) {
         int tv[UNR], alt[UNR], cur[UNR];
         bool ok[UNR];
-        // every load is unconditional (lanes past the end read edge e0) so that all of them
-        // are in flight together; only the atomics are guarded
 #pragma unroll
         for (int k = 0; k < UNR; ++k) {
             int e = base + t + k * stride;
@@ -190,7 +194,7 @@ __device__ __forceinline__ void relax_range(const DevState &st, const PushCtx &p
             alt[k] = du + (UNIT ? 1 : st.w[ec]);
         }
 #pragma unroll
-        for (int k = 0; k < UNR; ++k) cur[k] = ld_coherent(st.dist + tv[k]);
+        for (int k = 0; k < UNR; ++k) cur[k] = (GUARD && !ok[k]) ? 0 : ld_coherent(st.dist + tv[k]);
 #pragma unroll
         for (int k = 0; k < UNR; ++k) {
             bool want = false;
@@ -202,7 +206,7 @@ __device__ __forceinline__ void relax_range(const DevState &st, const PushCtx &p
 }
 
 // Lane bin: 64 frontier vertices (queue[qidx .. qidx+nvalid)), edges balanced over lanes.
-template <bool BATCH, bool UNIT, bool PUSH>
+template <bool BATCH, bool UNIT, bool PUSH, bool GUARD>
 __device__ __forceinline__ void lane_tile(const DevState &st, const PushCtx &pc, const int *q,
                                           long long qidx, int nvalid) {
     const int lane = lane_id();
@@ -249,7 +253,7 @@ __device__ __forceinline__ void lane_tile(const DevState &st, const PushCtx &pc,
             alt[k] = ou + (UNIT ? 1 : st.w[ec]);
         }
 #pragma unroll
-        for (int k = 0; k < UNR; ++k) cur[k] = ld_coherent(st.dist + tv[k]);
+        for (int k = 0; k < UNR; ++k) cur[k] = (GUARD && !ok[k]) ? 0 : ld_coherent(st.dist + tv[k]);
 #pragma unroll
         for (int k = 0; k < UNR; ++k) {
             bool want = false;
@@ -292,7 +296,7 @@ __device__ __noinline__ void build_tables(ItemTables &tb, const Cfg &cfg) {
     for (int t = 0; t < 4; ++t) tb.pre[t][NCLS] = acc[t];
 }
 
-template <bool BATCH, bool UNIT, bool PUSH>
+template <bool BATCH, bool UNIT, bool PUSH, bool GUARD>
 __device__ void expansion(const DevState &st, const PushCtx &pc, const int *q, ItemTables &tb) {
     const long long G = tb.pre[0][NCLS], K = tb.pre[1][NCLS], W = tb.pre[2][NCLS],
                     T = tb.pre[3][NCLS];
@@ -307,7 +311,7 @@ __device__ void expansion(const DevState &st, const PushCtx &pc, const int *q, I
         int e0 = st.row[VB + v], e1 = st.row[VB + v + 1];
         int du = ld_coherent(st.dist + slot);
         if (PUSH && threadIdx.x == 0) st_coherent_u(pc.bm_cur + (slot >> 5), 0u);
-        relax_range<UNIT, PUSH>(st, pc, B, e0, e1, du, threadIdx.x, WG);
+        relax_range<UNIT, PUSH, GUARD>(st, pc, B, e0, e1, du, threadIdx.x, WG);
     }
     // wavefront-granular items: grid chunks, wavefront items, lane tiles, round-robin
     const long long NW = (long long)gridDim.x * WAVES;
@@ -331,7 +335,7 @@ __device__ void expansion(const DevState &st, const PushCtx &pc, const int *q, I
             if (PUSH && j == 0 && lane_id() == 0) st_coherent_u(pc.bm_cur + (slot >> 5), 0u);
             if (e0 < e1) {
                 int du = ld_coherent(st.dist + slot);
-                relax_range<UNIT, PUSH>(st, pc, B, e0, e1, du, lane_id(), 64);
+                relax_range<UNIT, PUSH, GUARD>(st, pc, B, e0, e1, du, lane_id(), 64);
             }
         } else if (u < G + W) {
             long long i = u - G;
@@ -344,14 +348,14 @@ __device__ void expansion(const DevState &st, const PushCtx &pc, const int *q, I
             int e0 = st.row[VB + v], e1 = st.row[VB + v + 1];
             int du = ld_coherent(st.dist + slot);
             if (PUSH && lane_id() == 0) st_coherent_u(pc.bm_cur + (slot >> 5), 0u);
-            relax_range<UNIT, PUSH>(st, pc, B, e0, e1, du, lane_id(), 64);
+            relax_range<UNIT, PUSH, GUARD>(st, pc, B, e0, e1, du, lane_id(), 64);
         } else {
             long long t = u - G - W;
             int c = find_class(tb.pre[3], t);
             long long ti = t - tb.pre[3][c];
             long long first = ti * 64;
             int nvalid = (int)((long long)tb.cnt[c] - first < 64 ? (long long)tb.cnt[c] - first : 64);
-            lane_tile<BATCH, UNIT, PUSH>(st, pc, q, tb.seg[c] + first, nvalid);
+            lane_tile<BATCH, UNIT, PUSH, GUARD>(st, pc, q, tb.seg[c] + first, nvalid);
         }
     }
 }
@@ -552,7 +556,7 @@ __global__ __launch_bounds__(WG) void k_persistent(DevState st, SchedDev sp) {
         pc.bm_cur = st.bm[it & 1];
         pc.cnt_next = st.cnt + ((it + 1) % 3) * NCLS;
         pc.q_next = st.queue[(it + 1) & 1];
-        expansion<BATCH, UNIT, PUSH>(st, pc, q_cur, tb);
+        expansion<BATCH, UNIT, PUSH, false>(st, pc, q_cur, tb);
         if (!grid_sync(ctrl, nb, gen)) return;
     }
 }
@@ -645,7 +649,7 @@ __global__ __launch_bounds__(WG) void k_expand(DevState st, int it_off) {
     pc.bm_cur = st.bm[it & 1];
     pc.cnt_next = st.cnt + ((it + 1) % 3) * NCLS;
     pc.q_next = st.queue[(it + 1) & 1];
-    expansion<BATCH, UNIT, PUSH>(st, pc, st.queue[it & 1], tb);
+    expansion<BATCH, UNIT, PUSH, true>(st, pc, st.queue[it & 1], tb);
 }
 
 __global__ void k_advance(DevState st, int k) {
@@ -700,7 +704,7 @@ __device__ __forceinline__ void pk_relax_tail(const DevState &st, int v, int src
 }
 
 // Items of one queued vertex u, edges [e0, e1) x pending sources M; worker t of `stride`.
-template <bool UNIT>
+template <bool UNIT, bool GUARD>
 __device__ __forceinline__ void pk_relax_vertex(const DevState &st, int u, unsigned M, int e0, int e1,
                                                 int t, int stride) {
     const int P = __popc(M);
@@ -712,19 +716,19 @@ __device__ __forceinline__ void pk_relax_vertex(const DevState &st, int u, unsig
     while (e < e1) {
         int tv[UNR], alt[UNR], cur[UNR], src[UNR];
         bool ok[UNR];
-        // unconditional loads (lanes past the end read edge e0), guarded atomics
 #pragma unroll
         for (int k = 0; k < UNR; ++k) {
             ok[k] = e < e1;
             const int ec = ok[k] ? e : e0;
             src[k] = nth_set_bit(M, j);
             tv[k] = st.col[ec];
-            alt[k] = ld_coherent(du_row + src[k]) + (UNIT ? 1 : st.w[ec]);
+            const int du = ld_coherent(du_row + src[k]);  // in u's row for every j < P
+            alt[k] = (GUARD && !ok[k]) ? 0 : du + (UNIT ? 1 : st.w[ec]);
             e += qs; j += rs;
             if (j >= P) { j -= P; e += 1; }
         }
 #pragma unroll
-        for (int k = 0; k < UNR; ++k) cur[k] = ld_coherent(st.dist + ((long long)tv[k] << ks) + src[k]);
+        for (int k = 0; k < UNR; ++k) cur[k] = (GUARD && !ok[k]) ? 0 : ld_coherent(st.dist + ((long long)tv[k] << ks) + src[k]);
 #pragma unroll
         for (int k = 0; k < UNR; ++k)
             if (ok[k] && alt[k] < cur[k])
@@ -733,7 +737,7 @@ __device__ __forceinline__ void pk_relax_vertex(const DevState &st, int u, unsig
 }
 
 // Lane bin: 64 queued vertices, their items balanced over the lanes.
-template <bool UNIT>
+template <bool UNIT, bool GUARD>
 __device__ __forceinline__ void pk_lane_tile(const DevState &st, const int *q, const unsigned *qm,
                                              long long qidx, int nvalid) {
     const int lane = lane_id();
@@ -781,10 +785,10 @@ __device__ __forceinline__ void pk_lane_tile(const DevState &st, const int *q, c
             const int ec = ok[k] ? so + eo : e_safe;
             src[k] = ok[k] ? nth_set_bit(Mo, jo) : 0;
             tv[k] = st.col[ec];
-            alt[k] = ld_coherent(st.dist + ((long long)uo << ks) + src[k]) + (UNIT ? 1 : st.w[ec]);
+            alt[k] = (GUARD && !ok[k]) ? 0 : ld_coherent(st.dist + ((long long)uo << ks) + src[k]) + (UNIT ? 1 : st.w[ec]);
         }
 #pragma unroll
-        for (int k = 0; k < UNR; ++k) cur[k] = ld_coherent(st.dist + ((long long)tv[k] << ks) + src[k]);
+        for (int k = 0; k < UNR; ++k) cur[k] = (GUARD && !ok[k]) ? 0 : ld_coherent(st.dist + ((long long)tv[k] << ks) + src[k]);
 #pragma unroll
         for (int k = 0; k < UNR; ++k)
             if (ok[k] && alt[k] < cur[k])
@@ -792,7 +796,7 @@ __device__ __forceinline__ void pk_lane_tile(const DevState &st, const int *q, c
     }
 }
 
-template <bool UNIT>
+template <bool UNIT, bool GUARD>
 __device__ void pk_expansion(const DevState &st, const int *q, const unsigned *qm, ItemTables &tb) {
     const long long G = tb.pre[0][NCLS], K = tb.pre[1][NCLS], W = tb.pre[2][NCLS],
                     T = tb.pre[3][NCLS];
@@ -801,7 +805,7 @@ __device__ void pk_expansion(const DevState &st, const int *q, const unsigned *q
         long long qi = tb.seg[c] + (k - tb.pre[1][c]);
         int u = ld_coherent(q + qi);
         unsigned M = ld_coherent_u(qm + qi);
-        pk_relax_vertex<UNIT>(st, u, M, st.row[u], st.row[u + 1], threadIdx.x, WG);
+        pk_relax_vertex<UNIT, GUARD>(st, u, M, st.row[u], st.row[u + 1], threadIdx.x, WG);
     }
     const long long NW = (long long)gridDim.x * WAVES;
     const long long gw = (long long)blockIdx.x * WAVES + (threadIdx.x >> 6);
@@ -819,41 +823,67 @@ __device__ void pk_expansion(const DevState &st, const int *q, const unsigned *q
             int e0 = s + (j << GRID_CHUNK_SHIFT);
             int e1 = e0 + (1 << GRID_CHUNK_SHIFT);
             if (e1 > e) e1 = e;
-            pk_relax_vertex<UNIT>(st, u, M, e0, e1, lane_id(), 64);
+            pk_relax_vertex<UNIT, GUARD>(st, u, M, e0, e1, lane_id(), 64);
         } else if (x < G + W) {
             long long i = x - G;
             int c = find_class(tb.pre[2], i);
             long long qi = tb.seg[c] + (i - tb.pre[2][c]);
             int u = ld_coherent(q + qi);
             unsigned M = ld_coherent_u(qm + qi);
-            pk_relax_vertex<UNIT>(st, u, M, st.row[u], st.row[u + 1], lane_id(), 64);
+            pk_relax_vertex<UNIT, GUARD>(st, u, M, st.row[u], st.row[u + 1], lane_id(), 64);
         } else {
             long long t = x - G - W;
             int c = find_class(tb.pre[3], t);
             long long ti = t - tb.pre[3][c];
             long long first = ti * 64;
             int nvalid = (int)((long long)tb.cnt[c] - first < 64 ? (long long)tb.cnt[c] - first : 64);
-            pk_lane_tile<UNIT>(st, q, qm, tb.seg[c] + first, nvalid);
+            pk_lane_tile<UNIT, GUARD>(st, q, qm, tb.seg[c] + first, nvalid);
         }
     }
 }
 
-// The pending sources of v (mask m) whose distance at v is below the window T.
-__device__ __forceinline__ unsigned pk_take(const DevState &st, long long v, unsigned m, int T) {
+// The pending sources of v (mask m) whose distance at v is below the window T; *dmin
+// receives the least distance among the others.  The row is read with 16-byte loads
+// through the caches: a value older than the last atomicMin is only larger, so it can
+// only keep a pair pending one pass longer (every pending pair is expanded eventually).
+__device__ __forceinline__ unsigned pk_take(const DevState &st, long long v, unsigned m, int T, int *dmin) {
+    *dmin = NO_WINDOW;
     if (T == NO_WINDOW) return m;
     const int *row = st.dist + (v << st.kshift);
     unsigned take = 0;
-    for (unsigned b = m; b; b &= b - 1) {
-        const int i = __ffs(b) - 1;
-        if (ld_coherent(row + i) < T) take |= 1u << i;
+    int dm = NO_WINDOW;
+    if (st.kshift < 2) {  // rows of 1 or 2 distances
+        for (unsigned b = m; b; b &= b - 1) {
+            const int i = __ffs(b) - 1, d = row[i];
+            if (d < T) take |= 1u << i;
+            else dm = d < dm ? d : dm;
+        }
+    } else {
+        const int4 *row4 = (const int4 *)row;
+        const int nq = 1 << (st.kshift - 2);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            if (q >= nq || !((m >> (4 * q)) & 0xFu)) continue;
+            const int4 x = row4[q];
+            const int d[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (!((m >> (4 * q + k)) & 1u)) continue;
+                if (d[k] < T) take |= 1u << (4 * q + k);
+                else dm = d[k] < dm ? d[k] : dm;
+            }
+        }
     }
+    *dmin = dm;
     return take;
 }
 
 // Vertices with pending sources -> class-segmented queue with their source masks.  With a
 // priority window (T < NO_WINDOW) only the (vertex, source) pairs with dist < T are queued;
 // the others stay pending, and the pass adds their count and least distance to
-// defer_cnt / defer_min.
+// defer_cnt / defer_min.  With a window, the first pass over a bitmap word decides every
+// vertex (its take mask goes to mtake[v], its remaining mask back to mflag[v]); the second
+// writes the queue.  Without one every pending vertex is taken whole.
 __device__ __forceinline__ void pk_compaction(const DevState &st, int *cnt_out, int *q_out, unsigned *qm_out,
                                               int T, int *defer_cnt, int *defer_min) {
     __shared__ int s_lc[NCLS], s_lb[NCLS];
@@ -872,18 +902,24 @@ __device__ __forceinline__ void pk_compaction(const DevState &st, int *cnt_out, 
             const long long v = wi * 32 + b;
             const int c = st.cls[v];
             if (!c) continue;  // class 0 is never flagged; kept out regardless
+            if (T == NO_WINDOW) {  // every pending source is taken: mflag is read below
+                queued |= 1u << b;
+                atomicAdd(&s_lc[c], 1);
+                continue;
+            }
             const unsigned m = ld_coherent_u(st.mflag + v);
-            const unsigned t = pk_take(st, v, m, T);
-            if (t) { queued |= 1u << b; atomicAdd(&s_lc[c], 1); }
+            int dmin;
+            const unsigned t = pk_take(st, v, m, T, &dmin);
+            if (t) {
+                queued |= 1u << b;
+                atomicAdd(&s_lc[c], 1);
+                st.mtake[v] = t;  // read back by this thread below
+                st_coherent_u(st.mflag + v, m & ~t);
+            }
             if (t != m) {
                 keep |= 1u << b;
-                const unsigned rest = m & ~t;
-                my_dc += __popc(rest);
-                const int *row = st.dist + (v << st.kshift);
-                for (unsigned r = rest; r; r &= r - 1) {
-                    const int d = ld_coherent(row + (__ffs(r) - 1));
-                    my_dm = d < my_dm ? d : my_dm;
-                }
+                my_dc += __popc(m & ~t);
+                my_dm = dmin < my_dm ? dmin : my_dm;
             }
         }
         __syncthreads();
@@ -898,12 +934,16 @@ __device__ __forceinline__ void pk_compaction(const DevState &st, int *cnt_out, 
             const int c = st.cls[v];
             const int p = atomicAdd(&s_lc[c], 1);
             const long long qi = st.seg_off[c] + s_lb[c] + p;
-            const unsigned m = ld_coherent_u(st.mflag + v);
-            const unsigned t = pk_take(st, v, m, T);  // same answer as above: nothing writes dist now
+            unsigned t;
+            if (T == NO_WINDOW) {
+                t = ld_coherent_u(st.mflag + v);
+                st_coherent_u(st.mflag + v, 0u);
+            } else {
+                t = st.mtake[v];
+            }
             pairs += __popc(t);
             st_coherent(q_out + qi, (int)v);
             st_coherent_u(qm_out + qi, t);
-            st_coherent_u(st.mflag + v, m & ~t);
         }
         if (word != keep) st_coherent_u(bm + wi, keep);
         __syncthreads();
@@ -994,7 +1034,7 @@ __global__ __launch_bounds__(WG) void k_pk_persistent(DevState st, SchedDev sp) 
             record_trace(sp, it, cfg, tb.cnt, t_prev);
         }
         __syncthreads();
-        pk_expansion<UNIT>(st, q_cur, qm_cur, tb);
+        pk_expansion<UNIT, false>(st, q_cur, qm_cur, tb);
         if (!grid_sync(ctrl, nb, gen)) return;
     }
 }
@@ -1031,7 +1071,7 @@ __global__ __launch_bounds__(WG) void k_pk_expand(DevState st, int it_off) {
         build_tables(tb, pk_cfg(cfg));
     }
     __syncthreads();
-    pk_expansion<UNIT>(st, st.queue[it & 1], st.qmask[it & 1], tb);
+    pk_expansion<UNIT, true>(st, st.queue[it & 1], st.qmask[it & 1], tb);
 }
 
 __global__ void k_pk_init_sources(DevState st, const int *srcs, int nsrc, int T0) {
