@@ -696,11 +696,47 @@ __device__ __forceinline__ int nth_set_bit(unsigned m, int k) {
 // 64-vertex tile holds < 2^27 items (int) and item / P is exact in float arithmetic.
 constexpr int PACK_LANE_CLS = 17;
 
+// The relaxation of one item: atomicMin, then the improved source's bit goes to mflag[v] and
+// the first bit of mflag[v] sets v's frontier bit.
 __device__ __forceinline__ void pk_relax_tail(const DevState &st, int v, int src, int alt, int *p) {
     const int old = atomicMin(p, alt);
     if (alt >= old || st.cls[v] == 0) return;  // no improvement, or nothing to propagate
     const unsigned prev = atomicOr(&st.mflag[v], 1u << src);
     if (prev == 0) atomicOr(&st.bm[0][v >> 5], 1u << (v & 31));
+}
+
+// The relaxation of one item per lane (every lane of the wave calls it together): atomicMin
+// where alt improves on the loaded value, then the improved source bits go to mflag[v].  The
+// lanes of one edge are adjacent (items are edge-major), so several of them can improve the
+// same vertex at once; same-address atomics of one instruction are serialised, so a run of
+// adjacent lanes with the same target ORs its bits first and only the run's last lane updates.
+__device__ __forceinline__ void pk_relax_item(const DevState &st, bool live, int v, int src, int alt,
+                                              int cur, int ks) {
+    unsigned bit = 0;
+    if (live && alt < cur) {
+        const int old = atomicMin(st.dist + ((long long)v << ks) + src, alt);
+        if (alt < old && st.cls[v] != 0) bit = 1u << src;  // class 0: nothing to propagate
+    }
+    if (__ballot(bit != 0) == 0) return;
+    const int lane = lane_id();
+    // every lane takes part in the shuffle: a lane left out returns 0 to the lane reading it
+    const int vprev = __shfl_up(v, 1);
+    const bool head = lane == 0 || vprev != v;
+    if (__ballot(bit != 0 && !head) != 0) {
+        const unsigned long long heads = __ballot(head);
+        const int start = 63 - __clzll(heads & ((2ull << lane) - 1));  // lane 63: all ones
+        unsigned acc = bit;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const unsigned y = __shfl_up(acc, d);
+            if (lane - d >= start) acc |= y;
+        }
+        bit = (lane == 63 || ((heads >> (lane + 1)) & 1ull)) ? acc : 0u;
+    }
+    if (bit) {
+        const unsigned prev = atomicOr(&st.mflag[v], bit);
+        if (prev == 0) atomicOr(&st.bm[0][v >> 5], 1u << (v & 31));
+    }
 }
 
 // Items of one queued vertex u, edges [e0, e1) x pending sources M; worker t of `stride`.
@@ -713,7 +749,9 @@ __device__ __forceinline__ void pk_relax_vertex(const DevState &st, int u, unsig
     const int qs = stride / P, rs = stride - qs * P;  // one step of `stride` items
     int e = e0 + t / P, j = t - (t / P) * P;
     const int *du_row = st.dist + ((long long)u << ks);
-    while (e < e1) {
+    // The persistent kernel keeps every lane to the end (pk_relax_item is wave-wide); the
+    // per-iteration kernels let each lane leave and relax lane by lane (measured faster there).
+    while (GUARD ? e < e1 : __ballot(e < e1) != 0) {
         int tv[UNR], alt[UNR], cur[UNR], src[UNR];
         bool ok[UNR];
 #pragma unroll
@@ -730,9 +768,14 @@ __device__ __forceinline__ void pk_relax_vertex(const DevState &st, int u, unsig
 #pragma unroll
         for (int k = 0; k < UNR; ++k) cur[k] = (GUARD && !ok[k]) ? 0 : ld_coherent(st.dist + ((long long)tv[k] << ks) + src[k]);
 #pragma unroll
-        for (int k = 0; k < UNR; ++k)
-            if (ok[k] && alt[k] < cur[k])
-                pk_relax_tail(st, tv[k], src[k], alt[k], st.dist + ((long long)tv[k] << ks) + src[k]);
+        for (int k = 0; k < UNR; ++k) {
+            if (GUARD) {
+                if (ok[k] && alt[k] < cur[k])
+                    pk_relax_tail(st, tv[k], src[k], alt[k], st.dist + ((long long)tv[k] << ks) + src[k]);
+            } else {
+                pk_relax_item(st, ok[k], tv[k], src[k], alt[k], cur[k], ks);
+            }
+        }
     }
 }
 
@@ -785,14 +828,13 @@ __device__ __forceinline__ void pk_lane_tile(const DevState &st, const int *q, c
             const int ec = ok[k] ? so + eo : e_safe;
             src[k] = ok[k] ? nth_set_bit(Mo, jo) : 0;
             tv[k] = st.col[ec];
-            alt[k] = (GUARD && !ok[k]) ? 0 : ld_coherent(st.dist + ((long long)uo << ks) + src[k]) + (UNIT ? 1 : st.w[ec]);
+            const int du = ld_coherent(st.dist + ((long long)uo << ks) + src[k]);  // uo is a vertex
+            alt[k] = (GUARD && !ok[k]) ? 0 : du + (UNIT ? 1 : st.w[ec]);
         }
 #pragma unroll
         for (int k = 0; k < UNR; ++k) cur[k] = (GUARD && !ok[k]) ? 0 : ld_coherent(st.dist + ((long long)tv[k] << ks) + src[k]);
 #pragma unroll
-        for (int k = 0; k < UNR; ++k)
-            if (ok[k] && alt[k] < cur[k])
-                pk_relax_tail(st, tv[k], src[k], alt[k], st.dist + ((long long)tv[k] << ks) + src[k]);
+        for (int k = 0; k < UNR; ++k) pk_relax_item(st, ok[k], tv[k], src[k], alt[k], cur[k], ks);
     }
 }
 
