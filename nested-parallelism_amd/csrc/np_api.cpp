@@ -14,6 +14,7 @@
 #include <vector>
 #include <algorithm>
 #include <string>
+#include <functional>
 
 #include "np_internal.h"
 
@@ -74,7 +75,6 @@ struct Space {
     int *src_slots = nullptr;
     int src_cap = 0;
     unsigned *mflag = nullptr;    // source-packed: pending sources per vertex
-    unsigned *mtake = nullptr;    // source-packed: compaction scratch per vertex
     unsigned *qmask[2] = {nullptr, nullptr};
     // a batch part run concurrently with the others: its own control block, stream, event
     Ctrl *pctrl = nullptr;
@@ -95,7 +95,7 @@ struct Space {
         if (own_cls) dfree(cls);
         cls = nullptr;
         dfree(lut); dfree(inst); dfree(src_slots);
-        dfree(mflag); dfree(mtake); dfree(qmask[0]); dfree(qmask[1]);
+        dfree(mflag); dfree(qmask[0]); dfree(qmask[1]);
         dfree(pctrl);
         if (h_pctrl) (void)hipHostFree(h_pctrl);
         if (pstream) (void)hipStreamDestroy(pstream);
@@ -779,7 +779,6 @@ static int build_pack(np_ctx *c, np_batch *b, np_graph *g, const std::vector<int
     if (rc) return rc;
     HIPCHK(dalloc(&s.dist, (size_t)g->n << ks));
     HIPCHK(dalloc(&s.mflag, (size_t)g->n));
-    HIPCHK(dalloc(&s.mtake, (size_t)g->n));
     HIPCHK(dalloc(&s.qmask[0], (size_t)s.qcap));
     HIPCHK(dalloc(&s.qmask[1], (size_t)s.qcap));
     if ((rc = ensure_src(c, s, k))) return rc;
@@ -1015,7 +1014,6 @@ int np_batch_solve(np_ctx *c, np_batch *b, const np_sched *sched, np_stats *stat
         DevState st = make_state(c, s, g->srow, g->scol, unit ? nullptr : g->sw, s.dist, inf);
         st.kshift = pk->kshift;
         st.mflag = s.mflag;
-        st.mtake = s.mtake;
         st.qmask[0] = s.qmask[0]; st.qmask[1] = s.qmask[1];
         SchedDev sd = make_schedd(c, psch, g->n);
         if (psch.max_iters <= 0) {
@@ -1382,11 +1380,49 @@ static int sample_mean_weight(np_ctx *c, const np_graph *g, double *wmean) {
 static std::vector<int> window_candidates(double wmean) {
     std::vector<int> out;
     if (wmean <= 0) return out;
-    for (double f : {4.0, 16.0, 64.0}) {
+    for (double f : {2.0, 4.0, 8.0, 16.0, 64.0}) {
         const double d = f * wmean;
         if (d >= 1 && d < 1e9) out.push_back((int)d);
     }
     return out;
+}
+
+// Window stages of both autotuners.  `eval` times one schedule (ms).  The windows are tried on
+// the best schedule of the grid search; when one wins, the level and bin thresholds are searched
+// again under it on the same drive (a window changes which frontier sizes occur), and then half
+// and twice the window.
+static int tune_window(const std::function<int(const np_sched &, double *)> &eval, double wmean,
+                       np_sched *bs, double *bestt) {
+    const np_sched grid_best = *bs;
+    for (int delta : window_candidates(wmean)) {
+        np_sched s = grid_best;
+        s.delta = delta;
+        double t;
+        if (int rc = eval(s, &t)) return rc;
+        if (t < *bestt) { *bestt = t; *bs = s; }
+    }
+    if (bs->delta <= 0) return NP_OK;
+    const np_sched win = *bs;
+    for (int lv = 1; lv <= 2; ++lv)
+        for (int klo : {4, 5, 6, 7})
+            for (int khi : {8, 10, 12, 14, 30}) {
+                np_sched s = win;
+                s.level = lv; s.log2_t_lo = klo; s.log2_t_hi = khi;
+                if (s.level == win.level && klo == win.log2_t_lo && khi == win.log2_t_hi) continue;
+                double t;
+                if (int rc = eval(s, &t)) return rc;
+                if (t < *bestt) { *bestt = t; *bs = s; }
+            }
+    const np_sched th = *bs;
+    for (int delta : {th.delta / 2, th.delta * 2}) {
+        if (delta < 1 || delta >= 1000000000) continue;
+        np_sched s = th;
+        s.delta = delta;
+        double t;
+        if (int rc = eval(s, &t)) return rc;
+        if (t < *bestt) { *bestt = t; *bs = s; }
+    }
+    return NP_OK;
 }
 
 int np_autotune(np_ctx *c, np_graph *g, const int32_t *srcs, int32_t nsrc, const np_sched *base,
@@ -1432,18 +1468,18 @@ int np_autotune(np_ctx *c, np_graph *g, const int32_t *srcs, int32_t nsrc, const
     // priority window on top of the best scan-frontier schedule
     double wmean = 0;
     if (bs.frontier == NP_FRONTIER_SCAN && (rc = sample_mean_weight(c, g, &wmean)) != NP_OK) { dfree(d_dist); return rc; }
-    const np_sched grid_best = bs;
-    for (int delta : (bs.frontier == NP_FRONTIER_SCAN ? window_candidates(wmean) : std::vector<int>())) {
-        np_sched s = grid_best;
-        s.delta = delta;
-        double tot = 0;
-        for (int i = 0; i < nsrc && rc == NP_OK; ++i) {
-            np_stats st;
-            rc = solve_single(c, g, srcs[i], &s, d_dist, nullptr, false, &st);
-            tot += st.kernel_ms;
-        }
-        if (rc) { dfree(d_dist); return rc; }
-        if (tot < bestt) { bestt = tot; bs = s; }
+    if (bs.frontier == NP_FRONTIER_SCAN) {
+        auto eval = [&](const np_sched &s, double *ms) {
+            double tot = 0;
+            for (int i = 0; i < nsrc; ++i) {
+                np_stats st;
+                if (int e = solve_single(c, g, srcs[i], &s, d_dist, nullptr, false, &st)) return e;
+                tot += st.kernel_ms;
+            }
+            *ms = tot;
+            return (int)NP_OK;
+        };
+        if ((rc = tune_window(eval, wmean, &bs, &bestt))) { dfree(d_dist); return rc; }
     }
     dfree(d_dist);
     *best = bs;
@@ -1487,13 +1523,14 @@ int np_batch_autotune(np_ctx *c, np_batch *b, const np_sched *base, np_sched *be
     double wmean = 0;
     const np_graph *g0 = b->packs.empty() ? b->graphs[0] : b->packs[0]->g;
     if ((rc = sample_mean_weight(c, g0, &wmean))) return rc;
-    const np_sched grid_best = bs;
-    for (int delta : (b->unit ? std::vector<int>() : window_candidates(wmean))) {
-        np_sched s = grid_best;
-        s.delta = delta;
-        np_stats st;
-        if ((rc = np_batch_solve(c, b, &s, &st))) return rc;
-        if (st.kernel_ms < bestt) { bestt = st.kernel_ms; bs = s; }
+    if (!b->unit) {
+        auto eval = [&](const np_sched &s, double *ms) {
+            np_stats st;
+            if (int e = np_batch_solve(c, b, &s, &st)) return e;
+            *ms = st.kernel_ms;
+            return (int)NP_OK;
+        };
+        if ((rc = tune_window(eval, wmean, &bs, &bestt))) return rc;
     }
     *best = bs;
     if (best_ms) *best_ms = bestt;

@@ -52,7 +52,6 @@ struct DevState {
     // nonzero mflag; qmask[] = source mask of each queue entry
     int kshift;               // -1: one distance per slot
     unsigned *mflag;
-    unsigned *mtake;          // compaction scratch: the sources a queued vertex takes
     unsigned *qmask[2];
 };
 

@@ -368,6 +368,47 @@ __device__ void expansion(const DevState &st, const PushCtx &pc, const int *q, I
 // least distance among them to defer_cnt / defer_min.  T = INT_MAX queues every flag.
 constexpr int NO_WINDOW = 0x7FFFFFFF;
 
+// Single-source frontiers are sparse: a thread scans one bitmap word and visits its set bits,
+// so empty words cost one load.  Source-packed frontiers are dense (every vertex with a pending
+// source is flagged, and a priority window keeps deferred ones flagged): there each thread owns
+// CPT vertices of a tile of WG * CPT consecutive ones, so the lanes of a wave read the class
+// bytes, masks and distance rows of consecutive vertices and the 32 lanes of one bitmap word
+// rewrite it with one store.  Both count queue slots per class in LDS and reserve them once per
+// class per tile.
+constexpr int CPT = 4;
+
+// One bitmap word per 32 lanes: the first lane of each half-wave stores the kept bits when the
+// word changed.  v is this lane's vertex; its half-wave covers one word.
+__device__ __forceinline__ void rewrite_word(unsigned *bm, long long v, long long nv, unsigned word, bool keep) {
+    const unsigned nw = (unsigned)(__ballot(keep) >> (lane_id() & 32));
+    if ((lane_id() & 31) == 0 && v < nv && nw != word) st_coherent_u(bm + (v >> 5), nw);
+}
+
+// LDS class counters of this tile -> global reservation (s_lb) and reset for the next tile.
+__device__ __forceinline__ void reserve_classes(int *s_lc, int *s_lb, int *cnt_out) {
+    __syncthreads();
+    if (threadIdx.x < NCLS) {
+        const int k = s_lc[threadIdx.x];
+        s_lb[threadIdx.x] = k ? atomicAdd(&cnt_out[threadIdx.x], k) : 0;
+        s_lc[threadIdx.x] = 0;
+    }
+    __syncthreads();
+}
+
+// Deferred-pair statistics of the workgroup -> defer_cnt / defer_min.
+__device__ __forceinline__ void flush_deferred(int my_dc, int my_dm, int *s_dc, int *s_dm, int *defer_cnt,
+                                               int *defer_min) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        my_dc += __shfl_xor(my_dc, o);
+        const int x = __shfl_xor(my_dm, o);
+        my_dm = x < my_dm ? x : my_dm;
+    }
+    if (lane_id() == 0 && my_dc) { atomicAdd(s_dc, my_dc); atomicMin(s_dm, my_dm); }
+    __syncthreads();
+    if (threadIdx.x == 0 && *s_dc) { atomicAdd(defer_cnt, *s_dc); atomicMin(defer_min, *s_dm); }
+}
+
 __device__ __forceinline__ void compaction_impl(unsigned *bm, const unsigned char *cls, const int *seg_off,
                                                 long long nwords, int *cnt_out, int *q_out, const int *dist,
                                                 int T, int *defer_cnt, int *defer_min) {
@@ -413,17 +454,7 @@ __device__ __forceinline__ void compaction_impl(unsigned *bm, const unsigned cha
         if (take) st_coherent_u(bm + wi, word0 & ~take);
         __syncthreads();
     }
-    if (T != NO_WINDOW) {
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            my_dc += __shfl_xor(my_dc, o);
-            int t = __shfl_xor(my_dm, o);
-            my_dm = t < my_dm ? t : my_dm;
-        }
-        if (lane_id() == 0 && my_dc) { atomicAdd(&s_dc, my_dc); atomicMin(&s_dm, my_dm); }
-        __syncthreads();
-        if (threadIdx.x == 0 && s_dc) { atomicAdd(defer_cnt, s_dc); atomicMin(defer_min, s_dm); }
-    }
+    if (T != NO_WINDOW) flush_deferred(my_dc, my_dm, &s_dc, &s_dm, defer_cnt, defer_min);
 }
 
 // ------------------------------------------------------------------------------------
@@ -922,87 +953,64 @@ __device__ __forceinline__ unsigned pk_take(const DevState &st, long long v, uns
 
 // Vertices with pending sources -> class-segmented queue with their source masks.  With a
 // priority window (T < NO_WINDOW) only the (vertex, source) pairs with dist < T are queued;
-// the others stay pending, and the pass adds their count and least distance to
-// defer_cnt / defer_min.  With a window, the first pass over a bitmap word decides every
-// vertex (its take mask goes to mtake[v], its remaining mask back to mflag[v]); the second
-// writes the queue.  Without one every pending vertex is taken whole.
+// the others stay pending in mflag[v] (and v in the bitmap), and the pass adds their count and
+// least distance to defer_cnt / defer_min.
 __device__ __forceinline__ void pk_compaction(const DevState &st, int *cnt_out, int *q_out, unsigned *qm_out,
                                               int T, int *defer_cnt, int *defer_min) {
     __shared__ int s_lc[NCLS], s_lb[NCLS];
     __shared__ int s_dc, s_dm;
     unsigned *bm = st.bm[0];
-    int pairs = 0, my_dc = 0, my_dm = NO_WINDOW;
     if (threadIdx.x == 0) { s_dc = 0; s_dm = NO_WINDOW; }
-    for (long long wb = (long long)blockIdx.x * WG; wb < st.nwords; wb += (long long)gridDim.x * WG) {
-        if (threadIdx.x < NCLS) s_lc[threadIdx.x] = 0;
-        __syncthreads();
-        const long long wi = wb + threadIdx.x;
-        const unsigned word = wi < st.nwords ? ld_coherent_u(bm + wi) : 0u;
-        unsigned queued = 0, keep = 0;  // vertices of this word queued / still pending after the pass
-        for (unsigned w = word; w; w &= w - 1) {
-            const int b = __ffs(w) - 1;
-            const long long v = wi * 32 + b;
-            const int c = st.cls[v];
-            if (!c) continue;  // class 0 is never flagged; kept out regardless
-            if (T == NO_WINDOW) {  // every pending source is taken: mflag is read below
-                queued |= 1u << b;
-                atomicAdd(&s_lc[c], 1);
-                continue;
+    if (threadIdx.x < NCLS) s_lc[threadIdx.x] = 0;
+    __syncthreads();
+    int pairs = 0, my_dc = 0, my_dm = NO_WINDOW;
+    const long long nv = st.nwords * 32;
+    for (long long vb = (long long)blockIdx.x * (WG * CPT); vb < nv; vb += (long long)gridDim.x * (WG * CPT)) {
+        int qc[CPT], qp[CPT];
+        unsigned qt[CPT];
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) {
+            const long long v = vb + k * WG + threadIdx.x;
+            const unsigned word = v < nv ? ld_coherent_u(bm + (v >> 5)) : 0u;
+            bool keep = false;
+            qc[k] = 0;
+            qp[k] = 0;
+            qt[k] = 0;
+            if ((word >> (v & 31)) & 1u) {
+                const int c = st.cls[v];
+                if (c) {  // class 0 is never flagged; dropped regardless
+                    const unsigned m = ld_coherent_u(st.mflag + v);
+                    int dmin;
+                    const unsigned t = pk_take(st, v, m, T, &dmin);
+                    if (t) {
+                        qc[k] = c;
+                        qt[k] = t;
+                        qp[k] = atomicAdd(&s_lc[c], 1);
+                        st_coherent_u(st.mflag + v, m & ~t);
+                    }
+                    if (t != m) {
+                        keep = true;
+                        my_dc += __popc(m & ~t);
+                        my_dm = dmin < my_dm ? dmin : my_dm;
+                    }
+                }
             }
-            const unsigned m = ld_coherent_u(st.mflag + v);
-            int dmin;
-            const unsigned t = pk_take(st, v, m, T, &dmin);
-            if (t) {
-                queued |= 1u << b;
-                atomicAdd(&s_lc[c], 1);
-                st.mtake[v] = t;  // read back by this thread below
-                st_coherent_u(st.mflag + v, m & ~t);
-            }
-            if (t != m) {
-                keep |= 1u << b;
-                my_dc += __popc(m & ~t);
-                my_dm = dmin < my_dm ? dmin : my_dm;
-            }
+            rewrite_word(bm, v, nv, word, keep);
         }
-        __syncthreads();
-        if (threadIdx.x < NCLS) {
-            int k = s_lc[threadIdx.x];
-            s_lb[threadIdx.x] = k ? atomicAdd(&cnt_out[threadIdx.x], k) : 0;
-            s_lc[threadIdx.x] = 0;
+        reserve_classes(s_lc, s_lb, cnt_out);
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) {
+            if (!qc[k]) continue;
+            const long long qi = st.seg_off[qc[k]] + s_lb[qc[k]] + qp[k];
+            st_coherent(q_out + qi, (int)(vb + k * WG + threadIdx.x));
+            st_coherent_u(qm_out + qi, qt[k]);
+            pairs += __popc(qt[k]);
         }
-        __syncthreads();
-        for (unsigned w = queued; w; w &= w - 1) {
-            const long long v = wi * 32 + (__ffs(w) - 1);
-            const int c = st.cls[v];
-            const int p = atomicAdd(&s_lc[c], 1);
-            const long long qi = st.seg_off[c] + s_lb[c] + p;
-            unsigned t;
-            if (T == NO_WINDOW) {
-                t = ld_coherent_u(st.mflag + v);
-                st_coherent_u(st.mflag + v, 0u);
-            } else {
-                t = st.mtake[v];
-            }
-            pairs += __popc(t);
-            st_coherent(q_out + qi, (int)v);
-            st_coherent_u(qm_out + qi, t);
-        }
-        if (word != keep) st_coherent_u(bm + wi, keep);
-        __syncthreads();
     }
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        pairs += __shfl_xor(pairs, o);
-        my_dc += __shfl_xor(my_dc, o);
-        const int x = __shfl_xor(my_dm, o);
-        my_dm = x < my_dm ? x : my_dm;
-    }
+    for (int o = 32; o > 0; o >>= 1) pairs += __shfl_xor(pairs, o);
     if (lane_id() == 0 && pairs) atomicAdd((unsigned long long *)&st.ctrl->pairs_total, (unsigned long long)pairs);
-    if (T != NO_WINDOW) {
-        if (lane_id() == 0 && my_dc) { atomicAdd(&s_dc, my_dc); atomicMin(&s_dm, my_dm); }
-        __syncthreads();
-        if (threadIdx.x == 0 && s_dc) { atomicAdd(defer_cnt, s_dc); atomicMin(defer_min, s_dm); }
-    }
+    if (T != NO_WINDOW) flush_deferred(my_dc, my_dm, &s_dc, &s_dm, defer_cnt, defer_min);
 }
 
 // Lane-bin bound of a packed expansion (see PACK_LANE_CLS); the other bins keep theirs.
