@@ -368,13 +368,13 @@ __device__ void expansion(const DevState &st, const PushCtx &pc, const int *q, I
 // least distance among them to defer_cnt / defer_min.  T = INT_MAX queues every flag.
 constexpr int NO_WINDOW = 0x7FFFFFFF;
 
-// Single-source frontiers are sparse: a thread scans one bitmap word and visits its set bits,
-// so empty words cost one load.  Source-packed frontiers are dense (every vertex with a pending
-// source is flagged, and a priority window keeps deferred ones flagged): there each thread owns
-// CPT vertices of a tile of WG * CPT consecutive ones, so the lanes of a wave read the class
-// bytes, masks and distance rows of consecutive vertices and the 32 lanes of one bitmap word
-// rewrite it with one store.  Both count queue slots per class in LDS and reserve them once per
-// class per tile.
+// Single-source frontiers, and source-packed ones without a window, are sparse: a thread scans
+// one bitmap word and visits its set bits, so empty words cost one load.  With a priority window
+// a source-packed frontier is dense (deferred vertices stay flagged and are re-read every pass):
+// there each thread owns CPT vertices of a tile of WG * CPT consecutive ones, so the lanes of a
+// wave read the class bytes, masks and distance rows of consecutive vertices and the 32 lanes of
+// one bitmap word rewrite it with one store.  Both count queue slots per class in LDS and
+// reserve them once per class per tile.
 constexpr int CPT = 4;
 
 // One bitmap word per 32 lanes: the first lane of each half-wave stores the kept bits when the
@@ -965,7 +965,38 @@ __device__ __forceinline__ void pk_compaction(const DevState &st, int *cnt_out, 
     __syncthreads();
     int pairs = 0, my_dc = 0, my_dm = NO_WINDOW;
     const long long nv = st.nwords * 32;
-    for (long long vb = (long long)blockIdx.x * (WG * CPT); vb < nv; vb += (long long)gridDim.x * (WG * CPT)) {
+    // Without a window every pending vertex is taken whole and nothing stays flagged: the frontier
+    // is as sparse as a single source's, so a thread scans one bitmap word as compaction_impl does.
+    for (long long wb = (long long)blockIdx.x * WG; T == NO_WINDOW && wb < st.nwords;
+         wb += (long long)gridDim.x * WG) {
+        const long long wi = wb + threadIdx.x;
+        const unsigned word = wi < st.nwords ? ld_coherent_u(bm + wi) : 0u;
+        unsigned queued = 0;
+        for (unsigned w = word; w; w &= w - 1) {
+            const int b = __ffs(w) - 1;
+            const int c = st.cls[wi * 32 + b];
+            if (!c) continue;  // class 0 is never flagged; dropped regardless
+            queued |= 1u << b;
+            atomicAdd(&s_lc[c], 1);
+        }
+        reserve_classes(s_lc, s_lb, cnt_out);
+        for (unsigned w = queued; w; w &= w - 1) {
+            const long long v = wi * 32 + (__ffs(w) - 1);
+            const int c = st.cls[v];
+            const long long qi = st.seg_off[c] + s_lb[c] + atomicAdd(&s_lc[c], 1);
+            const unsigned t = ld_coherent_u(st.mflag + v);
+            st_coherent_u(st.mflag + v, 0u);
+            st_coherent(q_out + qi, (int)v);
+            st_coherent_u(qm_out + qi, t);
+            pairs += __popc(t);
+        }
+        if (word) st_coherent_u(bm + wi, 0u);
+        __syncthreads();
+        if (threadIdx.x < NCLS) s_lc[threadIdx.x] = 0;  // the second loop counted in s_lc
+        __syncthreads();
+    }
+    for (long long vb = (long long)blockIdx.x * (WG * CPT); T != NO_WINDOW && vb < nv;
+         vb += (long long)gridDim.x * (WG * CPT)) {
         int qc[CPT], qp[CPT];
         unsigned qt[CPT];
 #pragma unroll
