@@ -290,19 +290,33 @@ void np_host_free(void *p);
  * fmt 0 DIMACS9, 1 DIMACS10, 2 SLNDC (apps/sssp/src/gpu/sssp.cpp:120-125).  Same CSR as the
  * reference (per-vertex order reversed, DIMACS10/SLNDC weights rand()%1000+1 after
  * srand(7) in file order), built without adjacency lists.  *src gets the DIMACS9 's'
- * line (1-based in the file) or 0.  Arrays are freed with np_host_free. */
+ * line (1-based in the file) or 0.  fmt 3 = the binary CSR cache below (no counterpart in
+ * the reference).  Arrays are freed with np_host_free. */
 int np_read_graph(int32_t fmt, const char *path, int32_t *n, int64_t *m, int32_t *src,
                   int32_t **row, int32_t **col, int32_t **w);
 /* Same, from an open FILE* (passed as void*), as the reference readers use the global fp. */
 int np_read_graph_stream(int32_t fmt, void *fp, int32_t *n, int64_t *m, int32_t *src,
                          int32_t **row, int32_t **col, int32_t **w);
 
+/* ---- binary CSR cache -- no counterpart in the reference ----------------------------
+ * A graph read or generated once is saved and read back without parsing (the reference
+ * re-parses text on every run).  Layout, native little-endian: magic "NPCSR01\0", int64 n,
+ * int64 m, int32 source, int32 flags (bit 0: weights present), row[n+1], col[m], w[m] (int32).
+ * np_csr_load checks the sizes against the file and the CSR itself (row[0] = 0, row[n] = m,
+ * nondecreasing, col in [0, n)) and returns NP_ERR_IO otherwise; without weights it returns
+ * unit weights.  w may be NULL for np_csr_save.  Arrays are freed with np_host_free. */
+int np_csr_save(const char *path, int32_t n, int64_t m, int32_t src, const int32_t *row,
+                const int32_t *col, const int32_t *w);
+int np_csr_load(const char *path, int32_t *n, int64_t *m, int32_t *src, int32_t **row,
+                int32_t **col, int32_t **w);
+
 /* ---- autotuner -- no counterpart in the reference ----------------------------------
  * Times the candidate schedules (drive persistent or graph x level x log2_t_lo x
  * log2_t_hi x wgs_per_cu) on `g` from the given sources, then the best of them with a
- * priority window of 4, 16 and 64 times the mean edge weight, and returns the fastest in
- * *best (frontier mode and the other fields kept from `base`); *best_ms (optional) = its mean
- * device time per source. */
+ * priority window of 2, 4, 8, 16 and 64 times the mean edge weight; when a window wins, level
+ * and thresholds are searched again under it, then half and twice the window.  Returns the
+ * fastest in *best (frontier mode and the other fields kept from `base`); *best_ms
+ * (optional) = its mean device time per source. */
 int np_autotune(np_ctx *ctx, np_graph *g, const int32_t *srcs, int32_t nsrc, const np_sched *base,
                 np_sched *best, double *best_ms);
 /* The same search on a batch: each candidate times one np_batch_solve (all parts);

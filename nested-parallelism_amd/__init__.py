@@ -157,6 +157,10 @@ _SIGS = [
                                      ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
     ("np_read_graph_stream", ctypes.c_int, [ctypes.c_int32, _vp, _i32p, ctypes.POINTER(ctypes.c_int64), _i32p,
                                             ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
+    ("np_csr_save", ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_int32, _i32p, _i32p,
+                                   _i32p]),
+    ("np_csr_load", ctypes.c_int, [ctypes.c_char_p, _i32p, ctypes.POINTER(ctypes.c_int64), _i32p,
+                                   ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
     ("np_autotune", ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int32, ctypes.POINTER(Sched), ctypes.POINTER(Sched),
                                    ctypes.POINTER(ctypes.c_double)]),
     ("np_batch_autotune", ctypes.c_int, [_vp, _vp, ctypes.POINTER(Sched), ctypes.POINTER(Sched),
@@ -220,12 +224,36 @@ def default_sched(level=1):
 # host-only helpers (no device needed)
 # ---------------------------------------------------------------------------------------
 def read_graph(fmt, path):
-    """readInputDIMACS9/10/SLNDC + convertCSR (fmt 0/1/2).  Returns (row, col, w, source)."""
+    """readInputDIMACS9/10/SLNDC + convertCSR (fmt 0/1/2; 3 = the binary CSR cache).
+    Returns (row, col, w, source)."""
     L = load()
     n, m, s = ctypes.c_int32(), ctypes.c_int64(), ctypes.c_int32()
     r, c, w = _vp(), _vp(), _vp()
     _check(L.np_read_graph(fmt, os.fsencode(path), ctypes.byref(n), ctypes.byref(m), ctypes.byref(s),
                            ctypes.byref(r), ctypes.byref(c), ctypes.byref(w)))
+    return (_host_array(r, n.value + 1), _host_array(c, m.value), _host_array(w, m.value), s.value)
+
+
+def save_csr(path, row, col, w=None, source=0):
+    """Binary CSR cache (np_csr_save): row[n+1], col[m] and optional w[m] as int32."""
+    L = load()
+    row = np.ascontiguousarray(row, np.int32)
+    col = np.ascontiguousarray(col, np.int32)
+    wp = None
+    if w is not None:
+        w = np.ascontiguousarray(w, np.int32)
+        wp = w.ctypes.data_as(_i32p)
+    _check(L.np_csr_save(os.fsencode(path), len(row) - 1, len(col), source, row.ctypes.data_as(_i32p),
+                         col.ctypes.data_as(_i32p), wp))
+
+
+def load_csr(path):
+    """np_csr_load: returns (row, col, w, source); unit weights when the file has none."""
+    L = load()
+    n, m, s = ctypes.c_int32(), ctypes.c_int64(), ctypes.c_int32()
+    r, c, w = _vp(), _vp(), _vp()
+    _check(L.np_csr_load(os.fsencode(path), ctypes.byref(n), ctypes.byref(m), ctypes.byref(s),
+                         ctypes.byref(r), ctypes.byref(c), ctypes.byref(w)))
     return (_host_array(r, n.value + 1), _host_array(c, m.value), _host_array(w, m.value), s.value)
 
 

@@ -203,10 +203,88 @@ extern "C" {
 
 void np_host_free(void *p) { std::free(p); }
 
+// ---- binary CSR cache -------------------------------------------------------------------
+// A parsed or generated graph saved once and read back without parsing.  Layout (native
+// little-endian): 8-byte magic "NPCSR01\0", int64 n, int64 m, int32 source, int32 flags (bit 0:
+// weights present), then row[n + 1], col[m] and, with bit 0, w[m], all int32.  A read checks
+// the sizes against the file length and the CSR itself (row[0] = 0, row[n] = m, row
+// nondecreasing, every col in [0, n)) and fails with NP_ERR_IO otherwise.
+static const char kCsrMagic[8] = {'N', 'P', 'C', 'S', 'R', '0', '1', '\0'};
+
+struct CsrHeader {
+    char magic[8];
+    int64_t n, m;
+    int32_t src, flags;
+};
+
+static bool write_all(FILE *fp, const void *p, size_t bytes) { return bytes == 0 || std::fwrite(p, 1, bytes, fp) == bytes; }
+static bool read_all(FILE *fp, void *p, size_t bytes) { return bytes == 0 || std::fread(p, 1, bytes, fp) == bytes; }
+
+int np_csr_save(const char *path, int32_t n, int64_t m, int32_t src, const int32_t *row, const int32_t *col,
+                const int32_t *w) {
+    if (!path || n < 0 || m < 0 || !row || (m > 0 && !col)) { np::set_error("bad arguments"); return NP_ERR_ARG; }
+    FILE *fp = std::fopen(path, "wb");
+    if (!fp) { np::set_error("cannot open %s for writing", path); return NP_ERR_IO; }
+    CsrHeader h;
+    std::memcpy(h.magic, kCsrMagic, 8);
+    h.n = n; h.m = m; h.src = src; h.flags = w ? 1 : 0;
+    bool ok = write_all(fp, &h, sizeof h) && write_all(fp, row, sizeof(int32_t) * ((size_t)n + 1)) &&
+              write_all(fp, col, sizeof(int32_t) * (size_t)m) && (!w || write_all(fp, w, sizeof(int32_t) * (size_t)m));
+    ok = (std::fclose(fp) == 0) && ok;
+    if (!ok) { np::set_error("write to %s failed", path); return NP_ERR_IO; }
+    return NP_OK;
+}
+
+static int csr_load_stream(FILE *fp, int32_t *n, int64_t *m, int32_t *src, int32_t **row, int32_t **col,
+                           int32_t **w) {
+    CsrHeader h;
+    if (!read_all(fp, &h, sizeof h) || std::memcmp(h.magic, kCsrMagic, 8) != 0) {
+        np::set_error("not a binary CSR file"); return NP_ERR_IO;
+    }
+    if (h.n < 0 || h.n > 0x7FFFFFFE || h.m < 0 || h.m > 0x7FFFFFFF || (h.flags & ~1)) {
+        np::set_error("binary CSR header out of range"); return NP_ERR_IO;
+    }
+    const size_t nb = sizeof(int32_t) * ((size_t)h.n + 1), mb = sizeof(int32_t) * (size_t)h.m;
+    long here = std::ftell(fp);
+    if (here >= 0 && std::fseek(fp, 0, SEEK_END) == 0) {
+        const long end = std::ftell(fp);
+        if (end < 0 || (unsigned long)(end - here) != nb + mb * ((h.flags & 1) ? 2 : 1)) {
+            np::set_error("binary CSR file size does not match its header"); return NP_ERR_IO;
+        }
+        std::fseek(fp, here, SEEK_SET);
+    }
+    int32_t *r = (int32_t *)std::malloc(nb);
+    int32_t *c = (int32_t *)std::malloc(mb ? mb : 4);
+    int32_t *ww = (int32_t *)std::malloc(mb ? mb : 4);
+    if (!r || !c || !ww) { std::free(r); std::free(c); std::free(ww); np::set_error("host allocation failed"); return NP_ERR_NOMEM; }
+    bool ok = read_all(fp, r, nb) && read_all(fp, c, mb) && (!(h.flags & 1) || read_all(fp, ww, mb));
+    if (ok && !(h.flags & 1)) for (int64_t e = 0; e < h.m; ++e) ww[e] = 1;  // unweighted: unit weights
+    if (ok) ok = r[0] == 0 && (int64_t)r[h.n] == h.m;
+    for (int64_t v = 0; ok && v < h.n; ++v) ok = r[v] <= r[v + 1];
+    for (int64_t e = 0; ok && e < h.m; ++e) ok = c[e] >= 0 && (int64_t)c[e] < h.n;
+    if (!ok) {
+        std::free(r); std::free(c); std::free(ww);
+        np::set_error("binary CSR file is truncated or not a valid CSR"); return NP_ERR_IO;
+    }
+    *n = (int32_t)h.n; *m = h.m; *src = h.src; *row = r; *col = c; *w = ww;
+    return NP_OK;
+}
+
+int np_csr_load(const char *path, int32_t *n, int64_t *m, int32_t *src, int32_t **row, int32_t **col,
+                int32_t **w) {
+    if (!path || !n || !m || !src || !row || !col || !w) { np::set_error("null argument"); return NP_ERR_ARG; }
+    FILE *fp = std::fopen(path, "rb");
+    if (!fp) { np::set_error("cannot open %s", path); return NP_ERR_IO; }
+    int rc = csr_load_stream(fp, n, m, src, row, col, w);
+    std::fclose(fp);
+    return rc;
+}
+
 int np_read_graph_stream(int32_t fmt, void *stream, int32_t *n, int64_t *m, int32_t *src, int32_t **row,
                          int32_t **col, int32_t **w) {
     FILE *fp = (FILE *)stream;
     if (!fp || !n || !m || !src || !row || !col || !w) { np::set_error("null argument"); return NP_ERR_ARG; }
+    if (fmt == 3) return csr_load_stream(fp, n, m, src, row, col, w);
     EdgeList el;
     int nn = 0, s = 0, rc;
     switch (fmt) {
@@ -226,7 +304,7 @@ int np_read_graph_stream(int32_t fmt, void *stream, int32_t *n, int64_t *m, int3
 int np_read_graph(int32_t fmt, const char *path, int32_t *n, int64_t *m, int32_t *src, int32_t **row,
                   int32_t **col, int32_t **w) {
     if (!path) { np::set_error("null path"); return NP_ERR_ARG; }
-    FILE *fp = std::fopen(path, "r");
+    FILE *fp = std::fopen(path, fmt == 3 ? "rb" : "r");
     if (!fp) { np::set_error("cannot open %s", path); return NP_ERR_IO; }
     int rc = np_read_graph_stream(fmt, fp, n, m, src, row, col, w);
     std::fclose(fp);

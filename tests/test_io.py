@@ -84,6 +84,64 @@ def test_reader_errors(npa, tmp_path):
         npa.read_graph(7, str(p))
 
 
+def test_binary_csr_cache_round_trip(npa, tmp_path):
+    """np_csr_save / np_csr_load: a graph parsed from a reference format comes back identical,
+    through load_csr and through np_read_graph fmt 3; without weights it reads as unit weights;
+    the empty graph and a graph with isolated vertices round-trip too."""
+    g = load_golden("d9_rand.npz")
+    row, col, w, src = npa.read_graph(int(g["fmt"]), os.path.join(GOLDEN, str(g["path"])))
+    p = str(tmp_path / "g.npcsr")
+    npa.save_csr(p, row, col, w, source=src)
+    for got in (npa.load_csr(p), npa.read_graph(3, p)):
+        assert np.array_equal(got[0], row) and np.array_equal(got[1], col)
+        assert np.array_equal(got[2], w) and got[3] == src
+    assert os.path.getsize(p) == 32 + 4 * (len(row) + 2 * len(col))
+    npa.save_csr(p, row, col)                        # unweighted
+    r2, c2, w2, s2 = npa.load_csr(p)
+    assert np.array_equal(c2, col) and np.all(w2 == 1) and s2 == 0
+    for r, c in ((np.zeros(1, np.int32), np.zeros(0, np.int32)),           # n = 0, m = 0
+                 (np.array([0, 2, 2, 2, 3], np.int32), np.array([3, 1, 0], np.int32))):
+        npa.save_csr(p, r, c, np.arange(1, len(c) + 1, dtype=np.int32))
+        r3, c3, w3, _ = npa.load_csr(p)
+        assert np.array_equal(r3, r) and np.array_equal(c3, c) and np.array_equal(w3, np.arange(1, len(c) + 1))
+
+
+def test_binary_csr_cache_rejects_damaged_files(npa, tmp_path):
+    """Each damage is caught by the header, the file length or the CSR check: NP_ERR_IO."""
+    row = np.array([0, 2, 3, 3], np.int32)
+    col = np.array([1, 2, 0], np.int32)
+    w = np.array([5, 6, 7], np.int32)
+    good = str(tmp_path / "good.npcsr")
+    npa.save_csr(good, row, col, w)
+    data = open(good, "rb").read()
+    head = 32
+
+    def variant(name, blob):
+        q = str(tmp_path / name)
+        open(q, "wb").write(blob)
+        with pytest.raises(npa.NpError) as e:
+            npa.load_csr(q)
+        assert e.value.code == -8, name
+
+    variant("magic", b"X" + data[1:])
+    variant("truncated", data[:-4])
+    variant("longer", data + b"\0\0\0\0")
+    variant("header_only", data[:head])
+    bad = bytearray(data)
+    bad[head + 4:head + 8] = np.array([3], np.int32).tobytes()     # row = [0, 3, 2, 3]: decreasing
+    bad[head + 8:head + 12] = np.array([2], np.int32).tobytes()
+    variant("row_order", bytes(bad))
+    bad = bytearray(data)
+    bad[head + 16:head + 20] = np.array([7], np.int32).tobytes()   # col[0] = 7 >= n
+    variant("col_range", bytes(bad))
+    bad = bytearray(data)
+    bad[head + 12:head + 16] = np.array([2], np.int32).tobytes()   # row[n] != m
+    variant("row_end", bytes(bad))
+    with pytest.raises(npa.NpError) as e:
+        npa.load_csr(str(tmp_path / "missing.npcsr"))
+    assert e.value.code == -8
+
+
 def test_host_generators_deterministic_and_shaped(npa):
     a = npa.gen_rmat_host(11, 16, seed=5)
     b = npa.gen_rmat_host(11, 16, seed=5)
