@@ -298,6 +298,17 @@ int np_read_graph(int32_t fmt, const char *path, int32_t *n, int64_t *m, int32_t
 int np_read_graph_stream(int32_t fmt, void *fp, int32_t *n, int64_t *m, int32_t *src,
                          int32_t **row, int32_t **col, int32_t **w);
 
+/* ---- betweenness centrality: bc_np_consolidate_gpu + BC_GPU ----------------------------
+ * (apps/bc/src/gpu/bc_wrapper.cu:678-809, kernels bc_fp_kernel.cu:429-489, bc_bp_kernel.cu:4-36)
+ * Brandes' dependencies summed over the sources srcs[0..nsrc) (nsrc = 0: every vertex, as
+ * BC_GPU's loop does), unnormalised, on the directed graph with unit edge lengths; parallel
+ * edges count as distinct shortest paths, as the reference's per-edge path-count update does.
+ * bc_out[n] receives float values (the reference's float *bc), accumulated in double.  Sources
+ * run in batches of up to 32 on the same frontier machinery as the source-packed SSSP, host-loop
+ * driven (two 4-byte readbacks per level).  stats: iterations = levels summed over batches,
+ * frontier_total = (vertex, source mask) entries, kernel_ms = device time of the whole call. */
+int np_bc(np_ctx *ctx, np_graph *g, const int32_t *srcs, int32_t nsrc, float *bc_out, np_stats *stats);
+
 /* ---- binary CSR cache -- no counterpart in the reference ----------------------------
  * A graph read or generated once is saved and read back without parsing (the reference
  * re-parses text on every run).  Layout, native little-endian: magic "NPCSR01\0", int64 n,

@@ -157,6 +157,7 @@ _SIGS = [
                                      ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
     ("np_read_graph_stream", ctypes.c_int, [ctypes.c_int32, _vp, _i32p, ctypes.POINTER(ctypes.c_int64), _i32p,
                                             ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
+    ("np_bc", ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int32, _vp, ctypes.POINTER(Stats)]),
     ("np_csr_save", ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_int32, _i32p, _i32p,
                                    _i32p]),
     ("np_csr_load", ctypes.c_int, [ctypes.c_char_p, _i32p, ctypes.POINTER(ctypes.c_int64), _i32p,
@@ -366,6 +367,18 @@ class Context:
         _check(load().np_bfs(self.h, g.h, int(src), ctypes.byref(sched or Sched(level=2)), _ptr(lev),
                              ctypes.byref(st)))
         return lev, st
+
+    def bc(self, g, srcs=None):
+        """Betweenness centrality (BC_GPU, solution 7) summed over `srcs` (None: every vertex).
+        Returns (bc float32[n], Stats)."""
+        out = np.empty(g.n, np.float32)
+        st = Stats()
+        s = None if srcs is None else np.ascontiguousarray(srcs, np.int32)
+        ns = 0 if s is None else len(s)
+        if s is not None and ns == 0:
+            raise ValueError("empty source list (None means every vertex)")
+        _check(load().np_bc(self.h, g.h, _ptr(s), ns, _ptr(out), ctypes.byref(st)))
+        return out, st
 
     def reached_work(self, g, d_dist_ptr, inf=INF):
         a, b = ctypes.c_int64(), ctypes.c_int64()

@@ -24,7 +24,7 @@ HIP_FLAGS = ARCH + ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result",
                     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-I" + INC]
 HOST_FLAGS = HIP_FLAGS  # hipcc compiles .cpp as HIP as well: same target and flags
 
-HIP_SRC = ["np_kernels.hip", "np_gen.hip"]
+HIP_SRC = ["np_kernels.hip", "np_gen.hip", "np_bc.hip"]
 HOST_SRC = ["np_api.cpp", "np_graph_io.cpp"]
 
 

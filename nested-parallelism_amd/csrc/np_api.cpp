@@ -742,6 +742,137 @@ int np_reached_work(np_ctx *c, np_graph *g, const int32_t *d_dist, int32_t inf, 
     return NP_OK;
 }
 
+// ---- betweenness centrality ----------------------------------------------------------
+// bc_np_consolidate_gpu + BC_GPU (apps/bc/src/gpu/bc_wrapper.cu:678-809): sources in batches of
+// up to 32; per batch a level-synchronous forward BFS with path counts records every level's
+// entries, then the backward sweep replays them from the deepest level up.
+struct BcBufs {
+    int *lvl = nullptr, *ent_v = nullptr, *owner = nullptr, *count = nullptr, *srcs = nullptr;
+    unsigned *ent_m = nullptr, *nmask = nullptr;
+    double *sig = nullptr, *del = nullptr, *acc = nullptr;
+    long long *ent_c = nullptr, *cpref = nullptr, *h_total = nullptr;
+    float *out = nullptr;
+    void *temp = nullptr;
+    ~BcBufs() {
+        dfree(lvl); dfree(ent_v); dfree(owner); dfree(count); dfree(srcs); dfree(ent_m); dfree(nmask);
+        dfree(sig); dfree(del); dfree(acc); dfree(ent_c); dfree(cpref); dfree(out);
+        if (temp) (void)hipFree(temp);
+        if (h_total) (void)hipHostFree(h_total);
+    }
+};
+
+int np_bc(np_ctx *c, np_graph *g, const int32_t *srcs, int32_t nsrc, float *bc_out, np_stats *stats) {
+    if (!c || !g || g->ctx != c || !bc_out || nsrc < 0 || (nsrc > 0 && !srcs)) {
+        set_error("bad arguments"); return NP_ERR_ARG;
+    }
+    HIPCHK(hipSetDevice(c->device));
+    const int n = g->n;
+    std::vector<int> S;
+    if (nsrc == 0) {  // every vertex, as BC_GPU does (bc_wrapper.cu:692)
+        S.resize((size_t)n);
+        for (int i = 0; i < n; ++i) S[(size_t)i] = i;
+    } else {
+        S.assign(srcs, srcs + nsrc);
+        for (int i = 0; i < nsrc; ++i)
+            if (S[(size_t)i] < 0 || S[(size_t)i] >= n) { set_error("source %d out of range", S[(size_t)i]); return NP_ERR_ARG; }
+    }
+    np_stats st{};
+    st.drive = NP_DRIVE_HOSTLOOP;
+    if (n == 0) { if (stats) *stats = st; return NP_OK; }
+    const int P = (int)std::min<size_t>(32, S.size() ? S.size() : 1);
+    int ks = 0;
+    while ((1 << ks) < P) ++ks;
+    const long long cap_e = (long long)n * P;  // entries of one batch: at most one per (vertex, source)
+    if (cap_e > 0x7FFFFFFFll) { set_error("n x batch sources exceeds 2^31 entries"); return NP_ERR_CAPACITY; }
+    const long long cap_owner = g->m / 2 + (long long)n + 1;  // chunks of one level
+    if (cap_owner > 0x7FFFFFFFll) { set_error("too many chunks for one level"); return NP_ERR_CAPACITY; }
+    BcBufs bb;
+    const size_t slots = (size_t)n << ks;
+    HIPCHK(dalloc(&bb.lvl, slots));
+    HIPCHK(dalloc(&bb.sig, slots));
+    HIPCHK(dalloc(&bb.del, slots));
+    HIPCHK(dalloc(&bb.nmask, (size_t)n));
+    HIPCHK(dalloc(&bb.ent_v, (size_t)cap_e));
+    HIPCHK(dalloc(&bb.ent_m, (size_t)cap_e));
+    HIPCHK(dalloc(&bb.ent_c, (size_t)n + 1));
+    HIPCHK(dalloc(&bb.cpref, (size_t)n + 1));
+    HIPCHK(dalloc(&bb.owner, (size_t)cap_owner));
+    HIPCHK(dalloc(&bb.count, 1));
+    HIPCHK(dalloc(&bb.srcs, 32));
+    HIPCHK(dalloc(&bb.acc, (size_t)n));
+    HIPCHK(dalloc(&bb.out, (size_t)n));
+    HIPCHK(hipHostMalloc((void **)&bb.h_total, 2 * sizeof(long long)));
+    size_t temp_bytes = 0;
+    HIPCHK(bc_scan(nullptr, &temp_bytes, bb.ent_c, bb.cpref, n, c->stream));
+    HIPCHK(hipMalloc(&bb.temp, temp_bytes ? temp_bytes : 1));
+    HIPCHK(hipMemsetAsync(bb.acc, 0, sizeof(double) * (size_t)n, c->stream));
+    BcDev b{g->row, g->col, n, ks, bb.lvl, bb.sig, bb.del, bb.nmask};
+    // chunks of the ne entries at ev/em (ent_c holds their counts): prefix, owners; returns the total
+    auto chunks = [&](int ne, const int *ev, const unsigned *em, bool counted, long long *total) -> int {
+        if (!counted) HIPCHK(launch_bc_chunks(b, ev, em, ne, bb.ent_c, c->stream));
+        HIPCHK(hipMemsetAsync(bb.ent_c + ne, 0, sizeof(long long), c->stream));
+        size_t tb = temp_bytes;
+        HIPCHK(bc_scan(bb.temp, &tb, bb.ent_c, bb.cpref, ne, c->stream));
+        HIPCHK(hipMemcpyAsync(bb.h_total, bb.cpref + ne, sizeof(long long), hipMemcpyDeviceToHost, c->stream));
+        int rc = ctx_sync(c);
+        if (rc) return rc;
+        *total = bb.h_total[0];
+        if (*total < 0 || *total > cap_owner) { set_error("chunk count %lld out of range", *total); return NP_ERR_CAPACITY; }
+        HIPCHK(launch_bc_owner(bb.cpref, ne, bb.owner, c->stream));
+        return NP_OK;
+    };
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    std::vector<std::pair<long long, int>> levels;
+    for (size_t b0 = 0; b0 < S.size(); b0 += (size_t)P) {
+        const int np_ = (int)std::min<size_t>((size_t)P, S.size() - b0);
+        HIPCHK(hipMemcpyAsync(bb.srcs, S.data() + b0, sizeof(int) * (size_t)np_, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(launch_bc_reset(b, c->stream));
+        HIPCHK(launch_bc_sources(b, bb.srcs, np_, c->stream));
+        levels.clear();
+        long long E = 0;
+        for (int d = 0;; ++d) {
+            HIPCHK(hipMemsetAsync(bb.count, 0, sizeof(int), c->stream));
+            HIPCHK(launch_bc_compact(b, bb.ent_v + E, bb.ent_m + E, bb.ent_c, bb.count, c->stream));
+            HIPCHK(hipMemcpyAsync(bb.h_total + 1, bb.count, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+            int rc = ctx_sync(c);
+            if (rc) return rc;
+            int ne = 0;
+            std::memcpy(&ne, bb.h_total + 1, sizeof(int));
+            if (ne <= 0) break;
+            if (E + ne > cap_e) { set_error("entry capacity exceeded"); return NP_ERR_CAPACITY; }
+            levels.push_back({E, ne});
+            long long nch = 0;
+            if ((rc = chunks(ne, bb.ent_v + E, bb.ent_m + E, true, &nch))) return rc;
+            HIPCHK(launch_bc_expand(b, true, bb.ent_v + E, bb.ent_m + E, bb.cpref, bb.owner, nch, d, c->num_cus,
+                                    c->stream));
+            E += ne;
+        }
+        // the deepest level has no successors (dependency 0); level 0 holds the sources
+        for (int li = (int)levels.size() - 2; li >= 1; --li) {
+            const long long E0 = levels[(size_t)li].first;
+            const int ne = levels[(size_t)li].second;
+            long long nch = 0;
+            int rc = chunks(ne, bb.ent_v + E0, bb.ent_m + E0, false, &nch);
+            if (rc) return rc;
+            HIPCHK(launch_bc_expand(b, false, bb.ent_v + E0, bb.ent_m + E0, bb.cpref, bb.owner, nch, li,
+                                    c->num_cus, c->stream));
+            HIPCHK(launch_bc_accum(b, bb.ent_v + E0, bb.ent_m + E0, ne, bb.acc, c->stream));
+        }
+        st.iterations += (int)levels.size();
+        st.frontier_total += E;
+    }
+    HIPCHK(launch_bc_to_float(bb.acc, n, bb.out, c->stream));
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    HIPCHK(hipMemcpyAsync(bc_out, bb.out, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+    int rc = ctx_sync(c);
+    if (rc) return rc;
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    st.kernel_ms = ms;
+    if (stats) *stats = st;
+    return NP_OK;
+}
+
 int np_trace_get(np_ctx *c, np_trace_rec *recs, int32_t cap, int32_t *count) {
     if (!c || !count) { set_error("null argument"); return NP_ERR_ARG; }
     int k = std::min(c->last_iters, TRACE_CAP);

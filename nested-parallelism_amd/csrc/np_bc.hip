@@ -1,0 +1,212 @@
+// Betweenness centrality (Brandes) on the frontier machinery, up to 32 sources per batch.
+//
+// Reference: apps/bc/src/gpu/bc_wrapper.cu:678-761 (bc_np_consolidate_gpu: a level-synchronous
+// forward BFS that counts shortest paths, then a backward sweep over the levels) with the
+// kernels of bc_fp_kernel.cu:429-489 (forward: a vertex reached at level d+1 adds the path count
+// of every level-d predecessor, sigma[w] += sigma[u]) and bc_bp_kernel.cu:4-36 (backward:
+// delta[v] += sigma[v] / sigma[w] * (1 + delta[w]) over the edges v -> w into the next level;
+// bc[v] += delta[v] for v != source).  The reference keeps an n x n predecessor byte matrix;
+// here an edge u -> w is a predecessor edge exactly when level[w] = level[u] + 1, which is what
+// that matrix records, so no n^2 state exists.
+//
+// Layout: the sources of a batch share rows of 2^ks slots, slot (v << ks) + j for source j, as
+// the source-packed SSSP does: level (int), sigma and delta (double: path counts stay exact
+// integers up to 2^53).  A forward expansion of level d turns the entries (v, mask of the
+// sources for which v is at level d) into level d + 1 flags (nmask); the compaction writes the
+// next level's entries after the previous ones, so the backward sweep replays every level's
+// entries in reverse order.
+//
+// Work split: entry o has deg(v) x P' items, P' = popc(mask) rounded up to a power of two, item
+// r = (edge r / P', source slot r % P').  Items are cut into 64-item chunks (one wavefront each);
+// an owner array maps every chunk to its entry.  Chunk starts are multiples of 64, so a lane's
+// source slot is lane % P' and the lanes of one source are lane, lane ^ P', ...: the backward
+// sweep sums them with xor-shuffles before one atomicAdd per source.
+#include <hip/hip_runtime.h>
+#include <cstring>
+#include <rocprim/device/device_scan.hpp>
+#include "np_internal.h"
+
+namespace np {
+
+__device__ __forceinline__ int bc_lane() { return threadIdx.x & 63; }
+
+// k-th set bit of m (k < popc(m))
+__device__ __forceinline__ int bc_nth_bit(unsigned m, int k) {
+    for (int i = 0; i < k; ++i) m &= m - 1;
+    return __ffs(m) - 1;
+}
+
+__global__ void k_bc_reset(BcDev b) {
+    const long long slots = (long long)b.n << b.ks;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < slots;
+         i += (long long)gridDim.x * blockDim.x) {
+        b.lvl[i] = BC_UNSEEN;
+        b.sig[i] = 0.0;
+        b.del[i] = 0.0;
+        if (i < b.n) b.nmask[i] = 0u;
+    }
+}
+
+__global__ void k_bc_sources(BcDev b, const int *srcs, int nsrc) {
+    const int j = threadIdx.x;
+    if (blockIdx.x != 0 || j >= nsrc) return;
+    const int s = srcs[j];
+    const long long k = ((long long)s << b.ks) + j;
+    b.lvl[k] = 0;
+    b.sig[k] = 1.0;
+    atomicOr(&b.nmask[s], 1u << j);
+}
+
+// Flagged vertices -> entries (v, mask) at ent_v/ent_m + *count, and their chunk counts.
+__global__ void k_bc_compact(BcDev b, int *ent_v, unsigned *ent_m, long long *ent_c, int *count) {
+    for (long long base = (long long)blockIdx.x * blockDim.x; base < b.n; base += (long long)gridDim.x * blockDim.x) {
+        const long long v = base + threadIdx.x;
+        const unsigned m = v < b.n ? b.nmask[v] : 0u;
+        const unsigned long long want = __ballot(m != 0);
+        if (want == 0) continue;
+        const int lane = bc_lane();
+        const int leader = __ffsll((long long)want) - 1;
+        int pos0 = 0;
+        if (lane == leader) pos0 = atomicAdd(count, __popcll(want));
+        pos0 = __shfl(pos0, leader);
+        if (m) {
+            const int pos = pos0 + __popcll(want & ((1ull << lane) - 1));
+            b.nmask[v] = 0u;
+            const int p = __popc(m);
+            const int lp = p <= 1 ? 0 : 32 - __clz(p - 1);
+            const long long items = (long long)(b.row[v + 1] - b.row[v]) << lp;
+            ent_v[pos] = (int)v;
+            ent_m[pos] = m;
+            ent_c[pos] = (items + 63) >> 6;
+        }
+    }
+}
+
+// Chunk counts of ne entries (the backward sweep replays entries whose counts are gone).
+__global__ void k_bc_chunks(BcDev b, const int *ent_v, const unsigned *ent_m, int ne, long long *ent_c) {
+    for (long long o = (long long)blockIdx.x * blockDim.x + threadIdx.x; o < ne; o += (long long)gridDim.x * blockDim.x) {
+        const int v = ent_v[o];
+        const int p = __popc(ent_m[o]);
+        const int lp = p <= 1 ? 0 : 32 - __clz(p - 1);
+        ent_c[o] = (((long long)(b.row[v + 1] - b.row[v]) << lp) + 63) >> 6;
+    }
+}
+
+// owner[c] = o for every chunk c of entry o (cpref: exclusive prefix of chunk counts, ne + 1).
+__global__ void k_bc_owner(const long long *cpref, int ne, int *owner) {
+    for (long long o = (long long)blockIdx.x * blockDim.x + threadIdx.x; o < ne; o += (long long)gridDim.x * blockDim.x)
+        for (long long c = cpref[o]; c < cpref[o + 1]; ++c) owner[c] = (int)o;
+}
+
+// One 64-item chunk per wavefront.  FWD: level d -> d + 1 with path counts; else the
+// dependencies of level d from level d + 1.
+template <bool FWD>
+__global__ __launch_bounds__(256) void k_bc_expand(BcDev b, const int *ent_v, const unsigned *ent_m,
+                                                   const long long *cpref, const int *owner, long long nchunks,
+                                                   int d) {
+    const int lane = bc_lane();
+    const long long nw = (long long)gridDim.x * (blockDim.x >> 6);
+    for (long long c = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); c < nchunks; c += nw) {
+        const int o = owner[c];
+        const int v = ent_v[o];
+        const unsigned m = ent_m[o];
+        const int p = __popc(m);
+        const int lp = p <= 1 ? 0 : 32 - __clz(p - 1);
+        const int e0 = b.row[v];
+        const long long items = (long long)(b.row[v + 1] - e0) << lp;
+        const long long r = ((c - cpref[o]) << 6) + lane;
+        const int j = lane & ((1 << lp) - 1);
+        const bool live = r < items && j < p;
+        const int src = live ? bc_nth_bit(m, j) : 0;
+        const long long sv = ((long long)v << b.ks) + src;
+        if (FWD) {
+            if (live) {
+                const int w = b.col[e0 + (int)(r >> lp)];
+                const long long sw = ((long long)w << b.ks) + src;
+                const int old = atomicMin(&b.lvl[sw], d + 1);
+                if (old >= d + 1) {  // w is at level d + 1 for this source: v is a predecessor
+                    atomicAdd(&b.sig[sw], b.sig[sv]);
+                    if (old > d + 1) atomicOr(&b.nmask[w], 1u << src);  // first reached
+                }
+            }
+        } else {
+            double x = 0.0;
+            if (live) {
+                const int w = b.col[e0 + (int)(r >> lp)];
+                const long long sw = ((long long)w << b.ks) + src;
+                if (b.lvl[sw] == d + 1) x = b.sig[sv] / b.sig[sw] * (1.0 + b.del[sw]);
+            }
+            for (int off = 32; off >= (1 << lp) && off > 0; off >>= 1) x += __shfl_xor(x, off);
+            if (lane < p && lane == j && x != 0.0) atomicAdd(&b.del[((long long)v << b.ks) + bc_nth_bit(m, lane)], x);
+        }
+    }
+}
+
+// bc[v] += delta of v for every source of its entry at this level (level >= 1: v != source).
+__global__ void k_bc_accum(BcDev b, const int *ent_v, const unsigned *ent_m, int ne, double *bc) {
+    for (long long o = (long long)blockIdx.x * blockDim.x + threadIdx.x; o < ne; o += (long long)gridDim.x * blockDim.x) {
+        const int v = ent_v[o];
+        double s = 0.0;
+        for (unsigned m = ent_m[o]; m; m &= m - 1) s += b.del[((long long)v << b.ks) + (__ffs(m) - 1)];
+        bc[v] += s;  // one entry per vertex per level
+    }
+}
+
+__global__ void k_bc_to_float(const double *bc, int n, float *out) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        out[i] = (float)bc[i];
+}
+
+static int bc_grid(long long work, int cap) {
+    long long g = (work + 255) / 256;
+    return (int)(g < 1 ? 1 : (g > cap ? cap : g));
+}
+
+hipError_t launch_bc_reset(const BcDev &b, hipStream_t s) {
+    hipLaunchKernelGGL(k_bc_reset, dim3(bc_grid((long long)b.n << b.ks, 8192)), dim3(256), 0, s, b);
+    return hipGetLastError();
+}
+hipError_t launch_bc_sources(const BcDev &b, const int *srcs, int nsrc, hipStream_t s) {
+    hipLaunchKernelGGL(k_bc_sources, dim3(1), dim3(64), 0, s, b, srcs, nsrc);
+    return hipGetLastError();
+}
+hipError_t launch_bc_compact(const BcDev &b, int *ent_v, unsigned *ent_m, long long *ent_c, int *count,
+                             hipStream_t s) {
+    hipLaunchKernelGGL(k_bc_compact, dim3(bc_grid(b.n, 4096)), dim3(256), 0, s, b, ent_v, ent_m, ent_c, count);
+    return hipGetLastError();
+}
+hipError_t launch_bc_chunks(const BcDev &b, const int *ent_v, const unsigned *ent_m, int ne, long long *ent_c,
+                            hipStream_t s) {
+    hipLaunchKernelGGL(k_bc_chunks, dim3(bc_grid(ne, 4096)), dim3(256), 0, s, b, ent_v, ent_m, ne, ent_c);
+    return hipGetLastError();
+}
+hipError_t launch_bc_owner(const long long *cpref, int ne, int *owner, hipStream_t s) {
+    hipLaunchKernelGGL(k_bc_owner, dim3(bc_grid(ne, 4096)), dim3(256), 0, s, cpref, ne, owner);
+    return hipGetLastError();
+}
+hipError_t launch_bc_expand(const BcDev &b, bool fwd, const int *ent_v, const unsigned *ent_m,
+                            const long long *cpref, const int *owner, long long nchunks, int d, int num_cus,
+                            hipStream_t s) {
+    long long g = (nchunks + 3) / 4;  // 4 wavefronts per workgroup
+    const long long cap = (long long)num_cus * 16;
+    const int grid = (int)(g < 1 ? 1 : (g > cap ? cap : g));
+    if (fwd) hipLaunchKernelGGL(k_bc_expand<true>, dim3(grid), dim3(256), 0, s, b, ent_v, ent_m, cpref, owner, nchunks, d);
+    else hipLaunchKernelGGL(k_bc_expand<false>, dim3(grid), dim3(256), 0, s, b, ent_v, ent_m, cpref, owner, nchunks, d);
+    return hipGetLastError();
+}
+hipError_t launch_bc_accum(const BcDev &b, const int *ent_v, const unsigned *ent_m, int ne, double *bc,
+                           hipStream_t s) {
+    hipLaunchKernelGGL(k_bc_accum, dim3(bc_grid(ne, 4096)), dim3(256), 0, s, b, ent_v, ent_m, ne, bc);
+    return hipGetLastError();
+}
+hipError_t launch_bc_to_float(const double *bc, int n, float *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_bc_to_float, dim3(bc_grid(n, 4096)), dim3(256), 0, s, bc, n, out);
+    return hipGetLastError();
+}
+
+// Exclusive prefix of n chunk counts into out[0..n] (out[n] = total); temp may be null to query.
+hipError_t bc_scan(void *temp, size_t *temp_bytes, const long long *in, long long *out, int n, hipStream_t s) {
+    return rocprim::exclusive_scan(temp, *temp_bytes, in, out, 0ll, (size_t)n + 1, rocprim::plus<long long>(), s);
+}
+
+}  // namespace np

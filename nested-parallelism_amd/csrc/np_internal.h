@@ -120,6 +120,33 @@ hipError_t launch_mlp_loss_grad(const float *p, const float *x, const int *label
 hipError_t launch_adam(float *p, const float *grad, float *m, float *v, int n, float lr,
                        float b1, float b2, float eps, int step, hipStream_t s);
 
+// ---- bc.hip: betweenness centrality, up to 32 sources per batch -----------------------
+constexpr int BC_UNSEEN = 0x7FFFFFFF;
+struct BcDev {
+    const int *row, *col;  // CSR (original ids)
+    int n, ks;             // slots (v << ks) + j for batch source j < 2^ks
+    int *lvl;              // BFS level per slot, BC_UNSEEN when not reached
+    double *sig, *del;     // shortest-path counts and dependencies per slot
+    unsigned *nmask;       // [n] sources that first reached v in the last expansion
+};
+hipError_t launch_bc_reset(const BcDev &b, hipStream_t s);
+hipError_t launch_bc_sources(const BcDev &b, const int *srcs, int nsrc, hipStream_t s);
+// Flags -> entries (v, mask) at ent_v/ent_m[*count...] with chunk counts ent_c[*count...].
+hipError_t launch_bc_compact(const BcDev &b, int *ent_v, unsigned *ent_m, long long *ent_c, int *count,
+                             hipStream_t s);
+hipError_t launch_bc_chunks(const BcDev &b, const int *ent_v, const unsigned *ent_m, int ne, long long *ent_c,
+                            hipStream_t s);
+hipError_t launch_bc_owner(const long long *cpref, int ne, int *owner, hipStream_t s);
+hipError_t launch_bc_expand(const BcDev &b, bool fwd, const int *ent_v, const unsigned *ent_m,
+                            const long long *cpref, const int *owner, long long nchunks, int d, int num_cus,
+                            hipStream_t s);
+hipError_t launch_bc_accum(const BcDev &b, const int *ent_v, const unsigned *ent_m, int ne, double *bc,
+                           hipStream_t s);
+hipError_t launch_bc_to_float(const double *bc, int n, float *out, hipStream_t s);
+// Exclusive prefix of in[0..n] into out[0..n] (in[n] must be 0, so out[n] = total); temp = null
+// queries *temp_bytes.
+hipError_t bc_scan(void *temp, size_t *temp_bytes, const long long *in, long long *out, int n, hipStream_t s);
+
 // ---- gen.hip: generators and CSR construction ---------------------------------------
 struct GenOut { int *row; int *col; int *w; };
 // Builds CSR (reverse generation order per vertex) from an edge list in generation order.

@@ -30,6 +30,13 @@
  *                       the units of the algorithmic-byte count.
  *   orc_mlp_*           fp32 restatement of the scheduler MLP forward/backward; pinned
  *                       by finite differences in the tests.
+ *   orc_bc              Brandes betweenness, restating apps/bc/src/cpu/bc_kernel.cpp:20-95
+ *                       and bc.cpp:152-185 (and the GPU's bc_fp/bc_bp kernels): path counts
+ *                       over edges into the next BFS level, dependencies back from the
+ *                       deepest level, bc[v] += delta[v] for v != source; in double.  Pinned
+ *                       by the reference's bc_cpu compiled in oracle/_ref, closed forms
+ *                       (directed path, star, complete graph) and the definition
+ *                       sum over s != v != t of sigma_st(v) / sigma_st by brute force.
  *
  * Conventions follow common/global.h:19 (INF = 1073741824) and
  * apps/bfs-rec/src/gpu/bfs_rec.h:13 (UNDEFINED = 0x7FFFFFFF).
@@ -424,4 +431,46 @@ float orc_mlp_loss_grad(const float *p, const float *x, const int *label, const 
         }
     }
     return (float)(loss * inv);
+}
+
+/* Brandes betweenness summed over srcs[0..nsrc) (nsrc = 0: every vertex, as bc_cpu's loop
+ * over i does, bc.cpp:164): directed, unit edge lengths, unnormalised; a parallel edge is a
+ * distinct shortest path (sigma gains once per edge, bc_kernel.cpp:42-44).  Forward: queue BFS
+ * in level order, sigma[w] += sigma[u] for every edge u -> w with level[w] = level[u] + 1.
+ * Backward in reverse BFS order: delta[v] += sigma[v] / sigma[w] * (1 + delta[w]) over the same
+ * edges (bc_kernel.cpp:53-65), then bc[v] += delta[v] unless v is the source (:79-95). */
+int orc_bc(int n, const int *row, const int *col, const int *srcs, int nsrc, double *bc) {
+    int *lvl = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+    int *order = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+    double *sig = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    double *del = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    if (!lvl || !order || !sig || !del) { free(lvl); free(order); free(sig); free(del); return -1; }
+    for (int v = 0; v < n; ++v) bc[v] = 0.0;
+    const int ns = nsrc > 0 ? nsrc : n;
+    for (int k = 0; k < ns; ++k) {
+        const int s = nsrc > 0 ? srcs[k] : k;
+        if (s < 0 || s >= n) { free(lvl); free(order); free(sig); free(del); return -2; }
+        for (int v = 0; v < n; ++v) { lvl[v] = -1; sig[v] = 0.0; del[v] = 0.0; }
+        lvl[s] = 0; sig[s] = 1.0;
+        int head = 0, tail = 0;
+        order[tail++] = s;
+        while (head < tail) {
+            const int u = order[head++];
+            for (int e = row[u]; e < row[u + 1]; ++e) {
+                const int w = col[e];
+                if (lvl[w] < 0) { lvl[w] = lvl[u] + 1; order[tail++] = w; }
+                if (lvl[w] == lvl[u] + 1) sig[w] += sig[u];
+            }
+        }
+        for (int i = tail - 1; i >= 0; --i) {
+            const int v = order[i];
+            for (int e = row[v]; e < row[v + 1]; ++e) {
+                const int w = col[e];
+                if (lvl[w] == lvl[v] + 1) del[v] += sig[v] / sig[w] * (1.0 + del[w]);
+            }
+            if (v != s) bc[v] += del[v];
+        }
+    }
+    free(lvl); free(order); free(sig); free(del);
+    return 0;
 }

@@ -57,6 +57,8 @@ def lib():
                                        ctypes.POINTER(ll), ctypes.POINTER(ll)]
         L.orc_mlp_nparam.restype = ctypes.c_int
         L.orc_mlp_forward.restype = None
+        L.orc_bc.argtypes = [ctypes.c_int, _i32p, _i32p, _i32p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+        L.orc_bc.restype = ctypes.c_int
         L.orc_mlp_forward.argtypes = [_f32p, _f32p, ctypes.c_int, _f32p]
         L.orc_mlp_loss_grad.restype = ctypes.c_float
         L.orc_mlp_loss_grad.argtypes = [_f32p, _f32p, _i32p, _f32p, ctypes.c_int, ctypes.c_float, _f32p]
@@ -158,6 +160,20 @@ def reached_work(row, dist, inf=INF):
 MLP_NPARAM = 10 * 32 + 32 + 32 * 32 + 32 + 5 * 32 + 5
 
 
+def bc(row, col, srcs=None):
+    """Brandes betweenness (float64[n]) summed over srcs (None: every vertex)."""
+    row, col = _arr(row), _arr(col)
+    n = len(row) - 1
+    s = np.zeros(1, np.int32) if srcs is None else _arr(srcs)
+    ns = 0 if srcs is None else len(s)
+    out = np.zeros(max(n, 1), np.float64)
+    rc = lib().orc_bc(n, row, col if len(col) else np.zeros(1, np.int32), s if len(s) else np.zeros(1, np.int32),
+                      ns, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    if rc != 0:
+        raise ValueError("orc_bc failed: %d" % rc)
+    return out[:n]
+
+
 def mlp_forward(params, x):
     params = np.ascontiguousarray(params, np.float32)
     x = np.ascontiguousarray(x, np.float32)
@@ -240,6 +256,29 @@ def ref_read_graph(fmt, path):
 def ref_output_cost(path, src, dist):
     dist = _arr(dist)
     return _ref_sssp_lib().ref_output_cost(path.encode(), len(dist), int(src), dist)
+
+
+_ref_bc = None
+
+
+def ref_bc_cpu(row, col):
+    """The reference's bc_cpu (every vertex a source, float32) from oracle/_ref."""
+    global _ref_bc
+    if _ref_bc is None:
+        L = ctypes.CDLL(os.path.join(HERE, "_ref", "libref_bc.so"))
+        L.ref_bc_cpu.restype = ctypes.c_int
+        L.ref_bc_cpu.argtypes = [ctypes.c_int, ctypes.c_int, _i32p, _i32p, ctypes.POINTER(ctypes.c_float)]
+        _ref_bc = L
+    row, col = _arr(row), _arr(col)
+    n = len(row) - 1
+    out = np.zeros(max(n, 1), np.float32)
+    _ref_bc.ref_bc_cpu(n, len(col), row, col if len(col) else np.zeros(1, np.int32),
+                       out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+    return out[:n]
+
+
+def ref_bc_available():
+    return os.path.exists(os.path.join(HERE, "_ref", "libref_bc.so"))
 
 
 def ref_bfs_cpu(row, col, src):

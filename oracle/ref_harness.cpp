@@ -2,7 +2,8 @@
 //
 // A thin extern "C" harness linked against the reference's own CPU sources, compiled
 // from where they lie (common/global.cpp, common/heap.cpp, common/dheap.cpp,
-// apps/sssp/src/cpu/sssp.cpp; apps/bfs-rec/src/gpu/bfs_rec.cpp for the BFS build) by
+// apps/sssp/src/cpu/sssp.cpp; apps/bfs-rec/src/gpu/bfs_rec.cpp for the BFS build,
+// apps/bc/src/cpu/bc.cpp for the BC build) by
 // oracle/Makefile into oracle/_ref/.  It only sets the reference's globals and calls
 // its functions; it re-implements none of them.  The outputs (distances, pop counts,
 // CSR arrays of the reference readers) pin the C restatement in np_oracle.c and the
@@ -20,6 +21,14 @@ int SSSP_CPU();       // reference: apps/sssp/src/cpu/sssp.cpp:13
 #endif
 #ifdef NP_REF_BFS
 void bfs_cpu(int *levelArray);   // reference: apps/bfs-rec/src/gpu/bfs_rec.cpp:100
+#endif
+#ifdef NP_REF_BC
+#include <omp.h>
+void bc_cpu();                   // reference: apps/bc/src/cpu/bc.cpp:152
+extern int *sigma;               // reference globals: apps/bc/src/cpu/bc.h:31-37
+extern float *delta;
+extern char *p;
+extern int *p_idx, *p_data;
 #endif
 
 static void free_graph() {
@@ -158,6 +167,28 @@ int ref_bfs_cpu(int n, int m, const int *row, const int *col, int src, int *leve
     for (int i = 0; i < n; ++i) level[i] = 0x7FFFFFFF;
     level[src] = 0;
     bfs_cpu(level);
+    free_graph();
+    return 0;
+}
+#endif
+
+#ifdef NP_REF_BC
+// The reference bc_cpu() (apps/bc/src/cpu/bc.cpp:152-185 with bc_kernel.cpp): Brandes from every
+// vertex, float bc.  Its outer OpenMP loop over sources shares sigma/delta/p/levelArray between
+// threads, so it is run on one thread.  p is an n x n byte matrix: small graphs only.
+int ref_bc_cpu(int n, int m, const int *row, const int *col, float *out) {
+    set_graph(n, m, row, col, nullptr);
+    graph.frontier = new char[n > 0 ? n : 1]();
+    graph.update = new char[n > 0 ? n : 1]();
+    omp_set_num_threads(1);
+    bc_cpu();
+    std::memcpy(out, bc, sizeof(float) * n);
+    delete[] bc; bc = nullptr;
+    delete[] delta; delta = nullptr;
+    delete[] sigma; sigma = nullptr;
+    delete[] p; p = nullptr;
+    delete[] p_idx; p_idx = nullptr;
+    delete[] p_data; p_data = nullptr;
     free_graph();
     return 0;
 }
