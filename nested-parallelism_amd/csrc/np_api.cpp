@@ -747,14 +747,14 @@ int np_reached_work(np_ctx *c, np_graph *g, const int32_t *d_dist, int32_t inf, 
 // up to 32; per batch a level-synchronous forward BFS with path counts records every level's
 // entries, then the backward sweep replays them from the deepest level up.
 struct BcBufs {
-    int *lvl = nullptr, *ent_v = nullptr, *owner = nullptr, *count = nullptr, *srcs = nullptr;
+    int *lvl = nullptr, *ent_v = nullptr, *owner = nullptr, *count = nullptr, *srcs = nullptr, *err = nullptr;
     unsigned *ent_m = nullptr, *nmask = nullptr;
     double *sig = nullptr, *del = nullptr, *acc = nullptr;
     long long *ent_c = nullptr, *cpref = nullptr, *h_total = nullptr;
     float *out = nullptr;
     void *temp = nullptr;
     ~BcBufs() {
-        dfree(lvl); dfree(ent_v); dfree(owner); dfree(count); dfree(srcs); dfree(ent_m); dfree(nmask);
+        dfree(lvl); dfree(ent_v); dfree(owner); dfree(count); dfree(srcs); dfree(err); dfree(ent_m); dfree(nmask);
         dfree(sig); dfree(del); dfree(acc); dfree(ent_c); dfree(cpref); dfree(out);
         if (temp) (void)hipFree(temp);
         if (h_total) (void)hipHostFree(h_total);
@@ -798,6 +798,8 @@ int np_bc(np_ctx *c, np_graph *g, const int32_t *srcs, int32_t nsrc, float *bc_o
     HIPCHK(dalloc(&bb.cpref, (size_t)n + 1));
     HIPCHK(dalloc(&bb.owner, (size_t)cap_owner));
     HIPCHK(dalloc(&bb.count, 1));
+    HIPCHK(dalloc(&bb.err, 1));
+    HIPCHK(hipMemsetAsync(bb.err, 0, sizeof(int), c->stream));
     HIPCHK(dalloc(&bb.srcs, 32));
     HIPCHK(dalloc(&bb.acc, (size_t)n));
     HIPCHK(dalloc(&bb.out, (size_t)n));
@@ -807,18 +809,14 @@ int np_bc(np_ctx *c, np_graph *g, const int32_t *srcs, int32_t nsrc, float *bc_o
     HIPCHK(hipMalloc(&bb.temp, temp_bytes ? temp_bytes : 1));
     HIPCHK(hipMemsetAsync(bb.acc, 0, sizeof(double) * (size_t)n, c->stream));
     BcDev b{g->row, g->col, n, ks, bb.lvl, bb.sig, bb.del, bb.nmask};
-    // chunks of the ne entries at ev/em (ent_c holds their counts): prefix, owners; returns the total
-    auto chunks = [&](int ne, const int *ev, const unsigned *em, bool counted, long long *total) -> int {
+    // chunks of the ne entries at ev/em (ent_c holds their counts): prefix and owners, all on the
+    // stream (the expansion reads the chunk total from cpref[ne])
+    auto chunks = [&](int ne, const int *ev, const unsigned *em, bool counted) -> int {
         if (!counted) HIPCHK(launch_bc_chunks(b, ev, em, ne, bb.ent_c, c->stream));
         HIPCHK(hipMemsetAsync(bb.ent_c + ne, 0, sizeof(long long), c->stream));
         size_t tb = temp_bytes;
         HIPCHK(bc_scan(bb.temp, &tb, bb.ent_c, bb.cpref, ne, c->stream));
-        HIPCHK(hipMemcpyAsync(bb.h_total, bb.cpref + ne, sizeof(long long), hipMemcpyDeviceToHost, c->stream));
-        int rc = ctx_sync(c);
-        if (rc) return rc;
-        *total = bb.h_total[0];
-        if (*total < 0 || *total > cap_owner) { set_error("chunk count %lld out of range", *total); return NP_ERR_CAPACITY; }
-        HIPCHK(launch_bc_owner(bb.cpref, ne, bb.owner, c->stream));
+        HIPCHK(launch_bc_owner(bb.cpref, ne, bb.owner, cap_owner, bb.err, c->stream));
         return NP_OK;
     };
     HIPCHK(hipEventRecord(c->ev0, c->stream));
@@ -841,20 +839,18 @@ int np_bc(np_ctx *c, np_graph *g, const int32_t *srcs, int32_t nsrc, float *bc_o
             if (ne <= 0) break;
             if (E + ne > cap_e) { set_error("entry capacity exceeded"); return NP_ERR_CAPACITY; }
             levels.push_back({E, ne});
-            long long nch = 0;
-            if ((rc = chunks(ne, bb.ent_v + E, bb.ent_m + E, true, &nch))) return rc;
-            HIPCHK(launch_bc_expand(b, true, bb.ent_v + E, bb.ent_m + E, bb.cpref, bb.owner, nch, d, c->num_cus,
-                                    c->stream));
+            if ((rc = chunks(ne, bb.ent_v + E, bb.ent_m + E, true))) return rc;
+            HIPCHK(launch_bc_expand(b, true, bb.ent_v + E, bb.ent_m + E, bb.cpref, bb.owner, ne, cap_owner, d,
+                                    c->num_cus, c->stream));
             E += ne;
         }
         // the deepest level has no successors (dependency 0); level 0 holds the sources
         for (int li = (int)levels.size() - 2; li >= 1; --li) {
             const long long E0 = levels[(size_t)li].first;
             const int ne = levels[(size_t)li].second;
-            long long nch = 0;
-            int rc = chunks(ne, bb.ent_v + E0, bb.ent_m + E0, false, &nch);
+            int rc = chunks(ne, bb.ent_v + E0, bb.ent_m + E0, false);
             if (rc) return rc;
-            HIPCHK(launch_bc_expand(b, false, bb.ent_v + E0, bb.ent_m + E0, bb.cpref, bb.owner, nch, li,
+            HIPCHK(launch_bc_expand(b, false, bb.ent_v + E0, bb.ent_m + E0, bb.cpref, bb.owner, ne, cap_owner, li,
                                     c->num_cus, c->stream));
             HIPCHK(launch_bc_accum(b, bb.ent_v + E0, bb.ent_m + E0, ne, bb.acc, c->stream));
         }
@@ -864,8 +860,12 @@ int np_bc(np_ctx *c, np_graph *g, const int32_t *srcs, int32_t nsrc, float *bc_o
     HIPCHK(launch_bc_to_float(bb.acc, n, bb.out, c->stream));
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     HIPCHK(hipMemcpyAsync(bc_out, bb.out, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(bb.h_total, bb.err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     int rc = ctx_sync(c);
     if (rc) return rc;
+    int err = 0;
+    std::memcpy(&err, bb.h_total, sizeof(int));
+    if (err) { set_error("a level exceeded its chunk capacity"); return NP_ERR_CAPACITY; }
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     st.kernel_ms = ms;

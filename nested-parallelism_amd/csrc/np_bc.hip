@@ -92,19 +92,27 @@ __global__ void k_bc_chunks(BcDev b, const int *ent_v, const unsigned *ent_m, in
     }
 }
 
-// owner[c] = o for every chunk c of entry o (cpref: exclusive prefix of chunk counts, ne + 1).
-__global__ void k_bc_owner(const long long *cpref, int ne, int *owner) {
-    for (long long o = (long long)blockIdx.x * blockDim.x + threadIdx.x; o < ne; o += (long long)gridDim.x * blockDim.x)
-        for (long long c = cpref[o]; c < cpref[o + 1]; ++c) owner[c] = (int)o;
+// owner[c] = o for every chunk c of entry o (cpref: exclusive prefix of chunk counts, ne + 1),
+// one wavefront per entry.  A level has at most m/2 + ne chunks (deg x 32 / 64 + 1 per entry);
+// a total above cap sets *err and writes nothing past it.
+__global__ void k_bc_owner(const long long *cpref, int ne, int *owner, long long cap, int *err) {
+    const int lane = bc_lane();
+    const long long nw = (long long)gridDim.x * (blockDim.x >> 6);
+    if (blockIdx.x == 0 && threadIdx.x == 0 && cpref[ne] > cap) *err = 1;
+    for (long long o = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); o < ne; o += nw) {
+        const long long c1 = cpref[o + 1] < cap ? cpref[o + 1] : cap;
+        for (long long c = cpref[o] + lane; c < c1; c += 64) owner[c] = (int)o;
+    }
 }
 
 // One 64-item chunk per wavefront.  FWD: level d -> d + 1 with path counts; else the
 // dependencies of level d from level d + 1.
 template <bool FWD>
 __global__ __launch_bounds__(256) void k_bc_expand(BcDev b, const int *ent_v, const unsigned *ent_m,
-                                                   const long long *cpref, const int *owner, long long nchunks,
-                                                   int d) {
+                                                   const long long *cpref, const int *owner, int ne,
+                                                   long long cap, int d) {
     const int lane = bc_lane();
+    const long long nchunks = cpref[ne] < cap ? cpref[ne] : cap;
     const long long nw = (long long)gridDim.x * (blockDim.x >> 6);
     for (long long c = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); c < nchunks; c += nw) {
         const int o = owner[c];
@@ -180,18 +188,18 @@ hipError_t launch_bc_chunks(const BcDev &b, const int *ent_v, const unsigned *en
     hipLaunchKernelGGL(k_bc_chunks, dim3(bc_grid(ne, 4096)), dim3(256), 0, s, b, ent_v, ent_m, ne, ent_c);
     return hipGetLastError();
 }
-hipError_t launch_bc_owner(const long long *cpref, int ne, int *owner, hipStream_t s) {
-    hipLaunchKernelGGL(k_bc_owner, dim3(bc_grid(ne, 4096)), dim3(256), 0, s, cpref, ne, owner);
+hipError_t launch_bc_owner(const long long *cpref, int ne, int *owner, long long cap, int *err, hipStream_t s) {
+    const long long g = ((long long)ne + 3) / 4;  // 4 wavefronts per workgroup, one entry each
+    const int grid = (int)(g < 1 ? 1 : (g > 16384 ? 16384 : g));
+    hipLaunchKernelGGL(k_bc_owner, dim3(grid), dim3(256), 0, s, cpref, ne, owner, cap, err);
     return hipGetLastError();
 }
 hipError_t launch_bc_expand(const BcDev &b, bool fwd, const int *ent_v, const unsigned *ent_m,
-                            const long long *cpref, const int *owner, long long nchunks, int d, int num_cus,
+                            const long long *cpref, const int *owner, int ne, long long cap, int d, int num_cus,
                             hipStream_t s) {
-    long long g = (nchunks + 3) / 4;  // 4 wavefronts per workgroup
-    const long long cap = (long long)num_cus * 16;
-    const int grid = (int)(g < 1 ? 1 : (g > cap ? cap : g));
-    if (fwd) hipLaunchKernelGGL(k_bc_expand<true>, dim3(grid), dim3(256), 0, s, b, ent_v, ent_m, cpref, owner, nchunks, d);
-    else hipLaunchKernelGGL(k_bc_expand<false>, dim3(grid), dim3(256), 0, s, b, ent_v, ent_m, cpref, owner, nchunks, d);
+    const int grid = num_cus * 16;  // grid-stride over the chunks (their count is on the device)
+    if (fwd) hipLaunchKernelGGL(k_bc_expand<true>, dim3(grid), dim3(256), 0, s, b, ent_v, ent_m, cpref, owner, ne, cap, d);
+    else hipLaunchKernelGGL(k_bc_expand<false>, dim3(grid), dim3(256), 0, s, b, ent_v, ent_m, cpref, owner, ne, cap, d);
     return hipGetLastError();
 }
 hipError_t launch_bc_accum(const BcDev &b, const int *ent_v, const unsigned *ent_m, int ne, double *bc,

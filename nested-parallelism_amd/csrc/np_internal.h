@@ -136,9 +136,11 @@ hipError_t launch_bc_compact(const BcDev &b, int *ent_v, unsigned *ent_m, long l
                              hipStream_t s);
 hipError_t launch_bc_chunks(const BcDev &b, const int *ent_v, const unsigned *ent_m, int ne, long long *ent_c,
                             hipStream_t s);
-hipError_t launch_bc_owner(const long long *cpref, int ne, int *owner, hipStream_t s);
+// Chunk owners of ne entries; a chunk total above cap sets *err (nothing is written past cap).
+hipError_t launch_bc_owner(const long long *cpref, int ne, int *owner, long long cap, int *err, hipStream_t s);
+// Expansion of the chunks of ne entries (count cpref[ne], read on the device, at most cap).
 hipError_t launch_bc_expand(const BcDev &b, bool fwd, const int *ent_v, const unsigned *ent_m,
-                            const long long *cpref, const int *owner, long long nchunks, int d, int num_cus,
+                            const long long *cpref, const int *owner, int ne, long long cap, int d, int num_cus,
                             hipStream_t s);
 hipError_t launch_bc_accum(const BcDev &b, const int *ent_v, const unsigned *ent_m, int ne, double *bc,
                            hipStream_t s);
