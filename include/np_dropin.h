@@ -17,6 +17,8 @@
  *   CONF config, timing doubles     apps/sssp/src/gpu/sssp.h:13-30 (bfs_rec.h:15-32)
  *   SSSP_GPU                        apps/sssp/src/gpu/sssp_wrapper.cu:548-583
  *   BFS_REC_GPU                     apps/bfs-rec/src/gpu/bfs_rec_wrapper.cu:231-269
+ *   BC_GPU, float *bc               apps/bc/src/gpu/bc_wrapper.cu:783-809, common/global.cpp:15
+ *   outputBC                        common/global.cpp:377-383
  * The consolidation level is the CONSOLIDATE_LEVEL macro of the reference
  * (sssp_wrapper.cu:14-16, default 1 as in apps/sssp/src/gpu/Makefile:28; bfs_rec_wrapper.cu:
  * 20-22, default 2) given when libnpdropin.so is built, overridable at run time with the
@@ -81,5 +83,9 @@ int clear();
 
 void SSSP_GPU();
 void BFS_REC_GPU();
+
+extern float *bc;
+int outputBC(FILE *file);
+void BC_GPU();
 
 #endif /* NP_DROPIN_H */

@@ -1,9 +1,10 @@
 // np_cli.cpp -- command-line drivers with the reference's flags and output, over the
-// drop-in entry points.  Built twice: -DNP_CLI_SSSP gives gpu-sssp
+// drop-in entry points.  Built three times: -DNP_CLI_SSSP gives gpu-sssp
 // (apps/sssp/src/gpu/sssp.cpp:17-156), -DNP_CLI_BFS gives gpu-bfs-rec
-// (apps/bfs-rec/src/gpu/bfs_rec.cpp:17-217).  Flags: -h -v -d -f <fmt> -s <solution>
+// (apps/bfs-rec/src/gpu/bfs_rec.cpp:17-217), -DNP_CLI_BC gives gpu-bc
+// (apps/bc/src/gpu/bc.cpp:7-140).  Flags: -h -v -d -f <fmt> -s <solution>
 // -e <device> -i <file> (and -t, listed by the reference and ignored by its GPU binary).
-// Extra flag -o <file>: write the costs with outputCost after the runs.
+// Extra flag -o <file>: write the costs (levels, BC values) after the runs.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -17,8 +18,10 @@ static const char *g_out = nullptr;
 
 static void usage() {
     fprintf(stderr, "\n");
-#ifdef NP_CLI_SSSP
+#if defined(NP_CLI_SSSP)
     fprintf(stderr, "Usage:  gpu-sssp [option]\n");
+#elif defined(NP_CLI_BC)
+    fprintf(stderr, "Usage:  gpu-bc [option]\n");
 #else
     fprintf(stderr, "Usage:  gpu-bfs-rec [option]\n");
 #endif
@@ -42,6 +45,15 @@ static void usage() {
     fprintf(stderr, "                 4 - multiple dynamic parallelism per block\n");
     fprintf(stderr, "                 5 - single dynamic parallelism per block\n");
     fprintf(stderr, "                 6 - workload consolidation with dynamic parallelism\n");
+#elif defined(NP_CLI_BC)
+    fprintf(stderr, "                 0 - thread bitmap\n");
+    fprintf(stderr, "                 1 - thread queue\n");
+    fprintf(stderr, "                 2 - dual queue\n");
+    fprintf(stderr, "                 3 - shared delayed buffer\n");
+    fprintf(stderr, "                 4 - global delayed buffer\n");
+    fprintf(stderr, "                 5 - naive dp\n");
+    fprintf(stderr, "                 6 - optimized dp\n");
+    fprintf(stderr, "                 7 - workload consolidation for dynamic parallelism\n");
 #else
     fprintf(stderr, "                 0 - GPU flat\n");
     fprintf(stderr, "                 1 - GPU naive dynamic parallelism\n");
@@ -146,7 +158,17 @@ int main(int argc, char *argv[]) {
     convertCSR();
     t1 = gettime();
     if (VERBOSE) fprintf(stderr, "AdjList to CSR:\t\t%lf\n", t1 - t0);
-#ifdef NP_CLI_SSSP
+#if defined(NP_CLI_BC)
+    bc = new float[noNodeTotal > 0 ? noNodeTotal : 1];
+    BC_GPU();
+    if (DEBUG) outputBC(stdout);
+    if (g_out) {
+        FILE *f = fopen(g_out, "w");
+        if (f) { outputBC(f); fclose(f); }
+    }
+    delete[] bc;
+    bc = nullptr;
+#elif defined(NP_CLI_SSSP)
     for (int i = 0; i < N; ++i) SSSP_GPU();
     if (VERBOSE) {
         fprintf(stdout, "CUDA runtime initialization:\t\t%lf\n", init_time / N);

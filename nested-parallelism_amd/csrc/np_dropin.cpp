@@ -166,6 +166,15 @@ int convertCSR() {
     return 0;
 }
 
+float *bc = nullptr;  // common/global.cpp:15; the caller's main allocates n floats
+
+int outputBC(FILE *file) {
+    if (!file) file = stdout;
+    fprintf(stderr, "------- BC value ------\n");
+    for (int i = 0; i < noNodeTotal; ++i) fprintf(file, "Node %d: %f\n", i + 1, bc[i]);
+    return 0;
+}
+
 int outputCost(FILE *file) {
     if (!file) file = stdout;
     for (int i = 0; i < noNodeTotal; ++i)
@@ -234,5 +243,26 @@ void BFS_REC_GPU() {
     double t1 = gettime_ms();
     ker_exe_time += st.kernel_ms;
     d2h_memcpy_time += (t1 - t0) - st.kernel_ms > 0 ? (t1 - t0) - st.kernel_ms : 0;
+    np_graph_free(g);
+}
+
+// BC_GPU (apps/bc/src/gpu/bc_wrapper.cu:783-809): every vertex is a source.  Solutions 0-7
+// differ only in how frontier work is assigned to threads; each runs np_bc (7 is the
+// consolidated one).  Other solution numbers compute nothing and leave bc zero, as the
+// reference's switch does.  The timing lines are the reference's.
+void BC_GPU() {
+    np_ctx *c = ctx_for_device();
+    np_graph *g = upload_graph(c);
+    np_stats st;
+    memset(&st, 0, sizeof st);
+    start_time = gettime();
+    if (config.solution >= 0 && config.solution <= 7) {
+        if (np_bc(c, g, nullptr, 0, bc, &st) != NP_OK) fail("BC");
+    } else {
+        for (int i = 0; i < noNodeTotal; ++i) bc[i] = 0.f;
+    }
+    end_time = gettime();
+    fprintf(stdout, "Execution time:\t\t%lf\n", st.kernel_ms * 1e-3);
+    fprintf(stdout, "cudaMemcpy to CPU:\t\t%lf\n", (end_time - start_time) - st.kernel_ms * 1e-3);
     np_graph_free(g);
 }

@@ -1,6 +1,7 @@
 """The reference-named entry points (libnpdropin.so) through their command-line drivers:
 gpu-sssp / gpu-bfs-rec read the reference's file formats, call SSSP_GPU / BFS_REC_GPU N=10
-times and print what the reference's drivers print."""
+times and print what the reference's drivers print; gpu-bc calls BC_GPU once and prints the
+values with outputBC."""
 import os
 import subprocess
 
@@ -45,6 +46,22 @@ def test_gpu_sssp_cli_formats(tmp_path, name, solution):
     k = list(g["sources"]).index(src)
     expect = ["From Node %d -> %d is %d" % (src + 1, i + 1, d) for i, d in enumerate(g["dist"][k])]
     assert lines == expect
+
+
+@pytest.mark.parametrize("name", ["d9_rand", "d10_small"])
+def test_gpu_bc_cli_matches_oracle(tmp_path, orc, name):
+    """BC_GPU over every vertex (solution 7), values printed with outputBC's "%f"."""
+    g = load_golden(name + ".npz")
+    out = tmp_path / "bc.txt"
+    r = _run("gpu-bc", ["-f", str(int(g["fmt"])), "-i", os.path.join(GOLDEN, str(g["path"])), "-s", "7",
+                        "-o", str(out)], tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert "Execution time:" in r.stdout
+    lines = out.read_text().splitlines()
+    assert [l.split(":")[0] for l in lines] == ["Node %d" % (i + 1) for i in range(len(g["row"]) - 1)]
+    got = np.array([float(l.split(":")[1]) for l in lines])
+    ref = orc.bc(g["row"], g["col"])
+    assert np.allclose(got, ref, rtol=1e-5, atol=1e-6 + 5e-7), float(np.max(np.abs(got - ref)))
 
 
 @pytest.mark.parametrize("solution", ["0", "1", "2", "3"])
