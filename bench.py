@@ -16,7 +16,8 @@ The JSON value is the fastest mode; every mode is listed under "modes".
 The other configurations (--config c1|c2|c4|c5) are the parity/evaluation cases of the
 same path (ER 64K, R-MAT s22 per level, many sources + many uneven graphs, grid road
 graph); they print the same JSON schema.  --config bc measures betweenness centrality
-(BC_GPU) on the same frontier machinery (R-MAT s16, 1024 sources).
+(BC_GPU) on the same frontier machinery (R-MAT s16, 1024 sources); --config color the graph
+coloring of GRAPH_COLOR_GPU (R-MAT s20).
 
 The CPU leg (skipped with --no-cpu) is the checker and the CPU baseline: every distance of
 every instance and mode is checked with the shortest-path certificate, a bounded sample is
@@ -226,6 +227,69 @@ def bc_bench(npa, ctx, args):
     G.free()
 
 
+COLOR_METRIC = "graph-color GTEPS + %HBM peak @R-MAT s20, dependency counting vs the reference's rounds, exact"
+
+
+def color_bench(npa, ctx, args):
+    """--config color: GRAPH_COLOR_GPU on R-MAT scale 20 (1,048,576 vertices, 16,777,216 edges,
+    ids permuted).  Modes: `rounds` = the reference's algorithm (every uncolored vertex re-checks
+    its smaller neighbours each round), `depcount` = the dependency-counting frontier; both are
+    compared with the oracle (int32, exact).  GTEPS = m / step time.  Algorithmic bytes of
+    depcount = 8 per edge (col read by the count and the fill passes) + 12 per smaller-id edge
+    (its notify entry written and read, its counter decremented) + 40 per vertex (row bounds
+    twice 16, notify bounds twice 16, counter 4, color 4)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    G = npa.Graph.rmat(ctx, 20, 16, wmin=1, wmax=1, seed=7)
+    row, col, _ = G.download()
+    src = np.repeat(np.arange(G.n, dtype=np.int64), np.diff(row))
+    m_small = int((col < src).sum())
+    results = {}
+    outs = {}
+    for name, method in (("rounds", 1), ("depcount", 0)):
+        for _ in range(args.warmup):
+            ctx.color(G, method)
+        wall, dev = [], []
+        for _ in range(args.steps):
+            t = time.perf_counter()
+            outs[name], st = ctx.color(G, method)
+            wall.append((time.perf_counter() - t) * 1e3)
+            dev.append(st.kernel_ms)
+        ms = float(np.mean(wall))
+        results[name] = {"gteps": G.m / (ms * 1e-3) / 1e9, "ms_per_step": ms, "device_ms": float(np.mean(dev)),
+                         "rounds": int(st.iterations)}
+        log("%-9s %8.2f ms/step  %7.3f GTEPS  %d rounds" % (name, ms, results[name]["gteps"], st.iterations))
+    exact = cpu = None
+    if not args.no_cpu:
+        t = time.perf_counter()
+        ref = oracle.color(row, col)
+        tc = time.perf_counter() - t
+        exact = bool(all(np.array_equal(o, ref) for o in outs.values()))
+        cpu = {"value": round(G.m / tc / 1e9, 5), "unit": "GTEPS", "cores": 1, "kind": "port",
+               "sample": "oracle one-pass coloring (increasing ids) over the whole graph, one thread, %.2f s; "
+                         "both GPU methods equal it: %s" % (tc, exact)}
+    best = results["depcount"]
+    b_alg = 8.0 * G.m + 12.0 * m_small + 40.0 * G.n
+    achieved = b_alg / (best["device_ms"] * 1e-3) / 1e9
+    line = {
+        "metric": COLOR_METRIC, "value": round(best["gteps"], 3), "unit": "GTEPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(best["ms_per_step"], 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic",
+        "config": {"workload": "graph coloring (GRAPH_COLOR_GPU), R-MAT scale 20, edge factor 16, a=.57 b=.19 c=.19, "
+                               "ids permuted; one step colors the whole graph", "n": int(G.n), "m": int(G.m),
+                   "smaller_id_edges": m_small, "colors": int(outs["depcount"].max())},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_color_count+k_color_fill+k_color_expand (host loop)", "bytes_per_launch": b_alg},
+        "cpu_baseline": cpu, "exact": exact,
+        "modes": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                  for k, v in results.items()},
+        "speedup_depcount_vs_rounds": round(results["depcount"]["gteps"] / results["rounds"]["gteps"], 3),
+    }
+    print(json.dumps(line), flush=True)
+    G.free()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -255,6 +319,9 @@ def main():
     log("device: %d CUs, persistent occupancy %d workgroups/CU" % (ncu, occ))
     if args.config == "bc":
         bc_bench(npa, ctx, args)
+        return
+    if args.config == "color":
+        color_bench(npa, ctx, args)
         return
     t0 = time.time()
     W = build_workload(npa, ctx, args.config, args)

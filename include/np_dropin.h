@@ -19,6 +19,8 @@
  *   BFS_REC_GPU                     apps/bfs-rec/src/gpu/bfs_rec_wrapper.cu:231-269
  *   BC_GPU, float *bc               apps/bc/src/gpu/bc_wrapper.cu:783-809, common/global.cpp:15
  *   outputBC                        common/global.cpp:377-383
+ *   GRAPH_COLOR_GPU, outputColor    apps/graph-color/src/gpu/graph_color_wrapper.cu:662-699,
+ *                                   common/global.cpp:393-399 (colors in graph.colorArray)
  * The consolidation level is the CONSOLIDATE_LEVEL macro of the reference
  * (sssp_wrapper.cu:14-16, default 1 as in apps/sssp/src/gpu/Makefile:28; bfs_rec_wrapper.cu:
  * 20-22, default 2) given when libnpdropin.so is built, overridable at run time with the
@@ -87,5 +89,8 @@ void BFS_REC_GPU();
 extern float *bc;
 int outputBC(FILE *file);
 void BC_GPU();
+
+int outputColor(FILE *file);
+void GRAPH_COLOR_GPU();
 
 #endif /* NP_DROPIN_H */

@@ -309,6 +309,16 @@ int np_read_graph_stream(int32_t fmt, void *fp, int32_t *n, int64_t *m, int32_t 
  * frontier_total = (vertex, source mask) entries, kernel_ms = device time of the whole call. */
 int np_bc(np_ctx *ctx, np_graph *g, const int32_t *srcs, int32_t nsrc, float *bc_out, np_stats *stats);
 
+/* ---- graph coloring: GRAPH_COLOR_GPU --------------------------------------------------
+ * (apps/graph-color/src/gpu/graph_color_wrapper.cu:598-699, kernels graph_color_kernel.cu:
+ * 14-35, 779-834)  The reference's rounds give every vertex v
+ *     color[v] = 1 + max { color[u] : v -> u, u < v }   (1 when there is no such u),
+ * whatever the thread timing; color_out[n] receives these int32 values (graph.colorArray).
+ * method 0: dependency counting over the transposed smaller-neighbour lists, O(m) work;
+ * method 1: the reference's rounds as they are, O(rounds x m) (the baseline).
+ * stats: iterations = rounds, kernel_ms = device time of the whole call. */
+int np_color(np_ctx *ctx, np_graph *g, int32_t method, int32_t *color_out, np_stats *stats);
+
 /* ---- binary CSR cache -- no counterpart in the reference ----------------------------
  * A graph read or generated once is saved and read back without parsing (the reference
  * re-parses text on every run).  Layout, native little-endian: magic "NPCSR01\0", int64 n,

@@ -158,6 +158,7 @@ _SIGS = [
     ("np_read_graph_stream", ctypes.c_int, [ctypes.c_int32, _vp, _i32p, ctypes.POINTER(ctypes.c_int64), _i32p,
                                             ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
     ("np_bc", ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int32, _vp, ctypes.POINTER(Stats)]),
+    ("np_color", ctypes.c_int, [_vp, _vp, ctypes.c_int32, _vp, ctypes.POINTER(Stats)]),
     ("np_csr_save", ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_int32, _i32p, _i32p,
                                    _i32p]),
     ("np_csr_load", ctypes.c_int, [ctypes.c_char_p, _i32p, ctypes.POINTER(ctypes.c_int64), _i32p,
@@ -378,6 +379,14 @@ class Context:
         if s is not None and ns == 0:
             raise ValueError("empty source list (None means every vertex)")
         _check(load().np_bc(self.h, g.h, _ptr(s), ns, _ptr(out), ctypes.byref(st)))
+        return out, st
+
+    def color(self, g, method=0):
+        """Graph coloring of GRAPH_COLOR_GPU: method 0 dependency counting, 1 the reference's
+        rounds.  Returns (color int32[n], Stats)."""
+        out = np.empty(g.n, np.int32)
+        st = Stats()
+        _check(load().np_color(self.h, g.h, int(method), _ptr(out), ctypes.byref(st)))
         return out, st
 
     def reached_work(self, g, d_dist_ptr, inf=INF):

@@ -873,6 +873,113 @@ int np_bc(np_ctx *c, np_graph *g, const int32_t *srcs, int32_t nsrc, float *bc_o
     return NP_OK;
 }
 
+// ---- graph coloring --------------------------------------------------------------------
+// GRAPH_COLOR_GPU (apps/graph-color/src/gpu/graph_color_wrapper.cu:598-699): method 0 = the
+// dependency-counting frontier (O(m)), method 1 = the reference's rounds (O(rounds x m)).
+struct ColorBufs {
+    int *dep = nullptr, *cursor = nullptr, *ncol = nullptr, *q[3] = {nullptr, nullptr, nullptr}, *qn = nullptr,
+        *color = nullptr, *h = nullptr;
+    long long *ncnt = nullptr, *nrow = nullptr;
+    void *temp = nullptr;
+    hipStream_t cs = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    ~ColorBufs() {
+        dfree(dep); dfree(cursor); dfree(ncol); dfree(q[0]); dfree(q[1]); dfree(q[2]); dfree(qn); dfree(color);
+        dfree(ncnt); dfree(nrow);
+        if (temp) (void)hipFree(temp);
+        if (h) (void)hipHostFree(h);
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (cs) (void)hipStreamDestroy(cs);
+    }
+};
+static constexpr int COLOR_CHUNK = 30;  // rounds per captured graph (a multiple of the 3 rotating queues)
+
+int np_color(np_ctx *c, np_graph *g, int32_t method, int32_t *color_out, np_stats *stats) {
+    if (!c || !g || g->ctx != c || !color_out || method < 0 || method > 1) { set_error("bad arguments"); return NP_ERR_ARG; }
+    HIPCHK(hipSetDevice(c->device));
+    const int n = g->n;
+    np_stats st{};
+    st.drive = NP_DRIVE_HOSTLOOP;
+    if (n == 0) { if (stats) *stats = st; return NP_OK; }
+    ColorBufs cb;
+    HIPCHK(dalloc(&cb.color, (size_t)n));
+    HIPCHK(hipHostMalloc((void **)&cb.h, 2 * sizeof(int)));
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    int rounds = 0;
+    long long frontier_total = 0;
+    if (method == 1) {
+        HIPCHK(dalloc(&cb.qn, 1));
+        HIPCHK(hipMemsetAsync(cb.color, 0, sizeof(int) * (size_t)n, c->stream));
+        for (int col = 1;; ++col) {
+            HIPCHK(hipMemsetAsync(cb.qn, 0, sizeof(int), c->stream));
+            HIPCHK(launch_color_round(g->row, g->col, n, cb.color, col, cb.qn, c->stream));
+            HIPCHK(hipMemcpyAsync(cb.h, cb.qn, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+            int rc = ctx_sync(c);
+            if (rc) return rc;
+            ++rounds;
+            if (cb.h[0] == 0) break;  // nothing colored: every vertex has its color
+            if (col > n) { set_error("coloring did not finish in n rounds"); return NP_ERR_NOT_CONVERGED; }
+        }
+    } else {
+        HIPCHK(dalloc(&cb.dep, (size_t)n));
+        HIPCHK(dalloc(&cb.cursor, (size_t)n));
+        HIPCHK(dalloc(&cb.ncol, (size_t)(g->m > 0 ? g->m : 1)));
+        for (int k = 0; k < 3; ++k) HIPCHK(dalloc(&cb.q[k], (size_t)n));
+        HIPCHK(dalloc(&cb.qn, 4));  // three rotating frontier counts + the round counter
+        HIPCHK(dalloc(&cb.ncnt, (size_t)n + 1));
+        HIPCHK(dalloc(&cb.nrow, (size_t)n + 1));
+        size_t temp_bytes = 0;
+        HIPCHK(color_scan(nullptr, &temp_bytes, cb.ncnt, cb.nrow, n, c->stream));
+        HIPCHK(hipMalloc(&cb.temp, temp_bytes ? temp_bytes : 1));
+        HIPCHK(hipMemsetAsync(cb.ncnt, 0, sizeof(long long) * ((size_t)n + 1), c->stream));
+        HIPCHK(hipMemsetAsync(cb.cursor, 0, sizeof(int) * (size_t)n, c->stream));
+        HIPCHK(hipMemsetAsync(cb.qn, 0, 4 * sizeof(int), c->stream));
+        HIPCHK(launch_color_count(g->row, g->col, n, cb.dep, cb.ncnt, c->stream));
+        size_t tb = temp_bytes;
+        HIPCHK(color_scan(cb.temp, &tb, cb.ncnt, cb.nrow, n, c->stream));
+        HIPCHK(launch_color_fill(g->row, g->col, n, cb.nrow, cb.cursor, cb.ncol, c->stream));
+        HIPCHK(launch_color_init(cb.dep, n, cb.color, cb.q[0], cb.qn, c->stream));
+        // COLOR_CHUNK rounds captured once: round r reads queue r % 3 and fills (r + 1) % 3
+        HIPCHK(hipStreamCreateWithFlags(&cb.cs, hipStreamNonBlocking));
+        HIPCHK(hipStreamBeginCapture(cb.cs, hipStreamCaptureModeThreadLocal));
+        hipError_t e = hipSuccess;
+        for (int r = 0; r < COLOR_CHUNK && e == hipSuccess; ++r)
+            e = launch_color_step(cb.q[r % 3], cb.qn + r % 3, cb.q[(r + 1) % 3], cb.qn + (r + 1) % 3,
+                                  cb.qn + (r + 2) % 3, cb.nrow, cb.ncol, cb.dep, cb.color, cb.qn + 3, c->num_cus, cb.cs);
+        hipError_t e2 = hipStreamEndCapture(cb.cs, &cb.graph);
+        HIPCHK(e);
+        HIPCHK(e2);
+        HIPCHK(hipGraphInstantiate(&cb.gexec, cb.graph, nullptr, nullptr, 0));
+        for (long long done = 0;; done += COLOR_CHUNK) {
+            if (done > (long long)n + COLOR_CHUNK) { set_error("coloring did not finish in n rounds"); return NP_ERR_NOT_CONVERGED; }
+            HIPCHK(hipGraphLaunch(cb.gexec, c->stream));
+            HIPCHK(hipMemcpyAsync(cb.h, cb.qn, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));  // count 0
+            int rc = ctx_sync(c);
+            if (rc) return rc;
+            if (cb.h[0] == 0) break;  // the frontier of the next chunk's first round is empty
+        }
+        HIPCHK(hipMemcpyAsync(cb.h, cb.qn + 3, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        int rc = ctx_sync(c);
+        if (rc) return rc;
+        rounds = cb.h[0];
+        frontier_total = n;  // every vertex is in exactly one round's frontier
+        st.drive = NP_DRIVE_GRAPH;
+    }
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    HIPCHK(hipMemcpyAsync(color_out, cb.color, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+    int rc = ctx_sync(c);
+    if (rc) return rc;
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    st.kernel_ms = ms;
+    st.iterations = rounds;
+    st.frontier_total = frontier_total;
+    if (stats) *stats = st;
+    return NP_OK;
+}
+
 int np_trace_get(np_ctx *c, np_trace_rec *recs, int32_t cap, int32_t *count) {
     if (!c || !count) { set_error("null argument"); return NP_ERR_ARG; }
     int k = std::min(c->last_iters, TRACE_CAP);

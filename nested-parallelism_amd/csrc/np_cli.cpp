@@ -2,7 +2,8 @@
 // drop-in entry points.  Built three times: -DNP_CLI_SSSP gives gpu-sssp
 // (apps/sssp/src/gpu/sssp.cpp:17-156), -DNP_CLI_BFS gives gpu-bfs-rec
 // (apps/bfs-rec/src/gpu/bfs_rec.cpp:17-217), -DNP_CLI_BC gives gpu-bc
-// (apps/bc/src/gpu/bc.cpp:7-140).  Flags: -h -v -d -f <fmt> -s <solution>
+// (apps/bc/src/gpu/bc.cpp:7-140), -DNP_CLI_COLOR gives gpu-graph-color
+// (apps/graph-color/src/gpu/graph_color.cpp:7-160).  Flags: -h -v -d -f <fmt> -s <solution>
 // -e <device> -i <file> (and -t, listed by the reference and ignored by its GPU binary).
 // Extra flag -o <file>: write the costs (levels, BC values) after the runs.
 #include <cstdio>
@@ -22,6 +23,8 @@ static void usage() {
     fprintf(stderr, "Usage:  gpu-sssp [option]\n");
 #elif defined(NP_CLI_BC)
     fprintf(stderr, "Usage:  gpu-bc [option]\n");
+#elif defined(NP_CLI_COLOR)
+    fprintf(stderr, "Usage:  gpu-graph-color [option]\n");
 #else
     fprintf(stderr, "Usage:  gpu-bfs-rec [option]\n");
 #endif
@@ -54,6 +57,11 @@ static void usage() {
     fprintf(stderr, "                 5 - naive dp\n");
     fprintf(stderr, "                 6 - optimized dp\n");
     fprintf(stderr, "                 7 - workload consolidation for dynamic parallelism\n");
+#elif defined(NP_CLI_COLOR)
+    fprintf(stderr, "                 0-6   bitmap working set (no pruning ... optimized dp)\n");
+    fprintf(stderr, "                 7     bitmap + workload consolidation\n");
+    fprintf(stderr, "                 8-11  queue working set (delayed buffers, dp)\n");
+    fprintf(stderr, "                 12    queue + workload consolidation\n");
 #else
     fprintf(stderr, "                 0 - GPU flat\n");
     fprintf(stderr, "                 1 - GPU naive dynamic parallelism\n");
@@ -158,7 +166,17 @@ int main(int argc, char *argv[]) {
     convertCSR();
     t1 = gettime();
     if (VERBOSE) fprintf(stderr, "AdjList to CSR:\t\t%lf\n", t1 - t0);
-#if defined(NP_CLI_BC)
+#if defined(NP_CLI_COLOR)
+    graph.colorArray = new int[noNodeTotal > 0 ? noNodeTotal : 1]();
+    GRAPH_COLOR_GPU();
+    if (DEBUG) outputColor(stdout);
+    if (g_out) {
+        FILE *f = fopen(g_out, "w");
+        if (f) { outputColor(f); fclose(f); }
+    }
+    delete[] graph.colorArray;
+    graph.colorArray = nullptr;
+#elif defined(NP_CLI_BC)
     bc = new float[noNodeTotal > 0 ? noNodeTotal : 1];
     BC_GPU();
     if (DEBUG) outputBC(stdout);

@@ -175,6 +175,13 @@ int outputBC(FILE *file) {
     return 0;
 }
 
+int outputColor(FILE *file) {
+    if (!file) file = stdout;
+    fprintf(stderr, "------- Color value ------\n");
+    for (int i = 0; i < noNodeTotal; ++i) fprintf(file, "Node %d: %d\n", i + 1, graph.colorArray[i]);
+    return 0;
+}
+
 int outputCost(FILE *file) {
     if (!file) file = stdout;
     for (int i = 0; i < noNodeTotal; ++i)
@@ -264,5 +271,24 @@ void BC_GPU() {
     end_time = gettime();
     fprintf(stdout, "Execution time:\t\t%lf\n", st.kernel_ms * 1e-3);
     fprintf(stdout, "cudaMemcpy to CPU:\t\t%lf\n", (end_time - start_time) - st.kernel_ms * 1e-3);
+    np_graph_free(g);
+}
+
+// GRAPH_COLOR_GPU (apps/graph-color/src/gpu/graph_color_wrapper.cu:662-699): solutions 0-12
+// differ in how the rounds assign vertices to threads and all give the same colors; each runs
+// np_color (dependency counting).  Other solution numbers compute nothing, as the reference's
+// switch does.  The caller's main allocates graph.colorArray (graph_color.cpp:99-105).
+void GRAPH_COLOR_GPU() {
+    np_ctx *c = ctx_for_device();
+    np_graph *g = upload_graph(c);
+    np_stats st;
+    memset(&st, 0, sizeof st);
+    start_time = gettime();
+    if (config.solution >= 0 && config.solution <= 12) {
+        if (np_color(c, g, 0, graph.colorArray, &st) != NP_OK) fail("graph coloring");
+        if (DEBUG) fprintf(stderr, "Graph Coloring ends in %d iterations.\n", st.iterations);
+    }
+    end_time = gettime();
+    fprintf(stderr, "Execution time:\t\t%lf\n", end_time - start_time);
     np_graph_free(g);
 }

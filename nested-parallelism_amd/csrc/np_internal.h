@@ -149,6 +149,18 @@ hipError_t launch_bc_to_float(const double *bc, int n, float *out, hipStream_t s
 // queries *temp_bytes.
 hipError_t bc_scan(void *temp, size_t *temp_bytes, const long long *in, long long *out, int n, hipStream_t s);
 
+// ---- color.hip: graph coloring (longest decreasing-id path labels) -----------------------
+hipError_t launch_color_count(const int *row, const int *col, int n, int *dep, long long *ncnt, hipStream_t s);
+hipError_t launch_color_fill(const int *row, const int *col, int n, const long long *nrow, int *cursor, int *ncol,
+                             hipStream_t s);
+hipError_t launch_color_init(const int *dep, int n, int *color, int *q, int *qn, hipStream_t s);
+// One dependency-counting round (graph-capturable: no host values depend on the frontier).
+hipError_t launch_color_step(const int *q_in, const int *qn_in, int *q_out, int *qn_out, int *qn_zero,
+                             const long long *nrow, const int *ncol, int *dep, int *color, int *rounds, int num_cus,
+                             hipStream_t s);
+hipError_t launch_color_round(const int *row, const int *col, int n, int *color, int c, int *nonstop, hipStream_t s);
+hipError_t color_scan(void *temp, size_t *temp_bytes, const long long *in, long long *out, int n, hipStream_t s);
+
 // ---- gen.hip: generators and CSR construction ---------------------------------------
 struct GenOut { int *row; int *col; int *w; };
 // Builds CSR (reverse generation order per vertex) from an edge list in generation order.

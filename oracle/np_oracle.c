@@ -30,6 +30,12 @@
  *                       the units of the algorithmic-byte count.
  *   orc_mlp_*           fp32 restatement of the scheduler MLP forward/backward; pinned
  *                       by finite differences in the tests.
+ *   orc_color           the colors of the reference's graph-coloring rounds
+ *                       (apps/graph-color/src/gpu/graph_color_kernel.cu:14-35):
+ *                       color[v] = 1 + max color of the out-neighbours with a smaller id.
+ *                       Pinned by a simulation of the rounds themselves in random thread
+ *                       orders (tests), closed forms (paths, complete digraph) and, on
+ *                       symmetric graphs, the proper-coloring invariant.
  *   orc_bc              Brandes betweenness, restating apps/bc/src/cpu/bc_kernel.cpp:20-95
  *                       and bc.cpp:152-185 (and the GPU's bc_fp/bc_bp kernels): path counts
  *                       over edges into the next BFS level, dependencies back from the
@@ -473,4 +479,17 @@ int orc_bc(int n, const int *row, const int *col, const int *srcs, int nsrc, dou
     }
     free(lvl); free(order); free(sig); free(del);
     return 0;
+}
+
+/* The reference's graph coloring (graph_color_wrapper.cu:598-635): round c colors every
+ * uncolored v none of whose out-neighbours u < v is uncolored or colored c.  In increasing id
+ * order every smaller neighbour already has its final color, so one pass gives it:
+ * color[v] = 1 + max color[u] over the edges v -> u with u < v (self loops have u == v). */
+void orc_color(int n, const int *row, const int *col, int *color) {
+    for (int v = 0; v < n; ++v) {
+        int c = 0;
+        for (int e = row[v]; e < row[v + 1]; ++e)
+            if (col[e] < v && color[col[e]] > c) c = color[col[e]];
+        color[v] = c + 1;
+    }
 }

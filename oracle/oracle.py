@@ -57,6 +57,8 @@ def lib():
                                        ctypes.POINTER(ll), ctypes.POINTER(ll)]
         L.orc_mlp_nparam.restype = ctypes.c_int
         L.orc_mlp_forward.restype = None
+        L.orc_color.argtypes = [ctypes.c_int, _i32p, _i32p, _i32p]
+        L.orc_color.restype = None
         L.orc_bc.argtypes = [ctypes.c_int, _i32p, _i32p, _i32p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
         L.orc_bc.restype = ctypes.c_int
         L.orc_mlp_forward.argtypes = [_f32p, _f32p, ctypes.c_int, _f32p]
@@ -158,6 +160,15 @@ def reached_work(row, dist, inf=INF):
 
 
 MLP_NPARAM = 10 * 32 + 32 + 32 * 32 + 32 + 5 * 32 + 5
+
+
+def color(row, col):
+    """Colors of the reference's graph-coloring rounds (int32[n])."""
+    row, col = _arr(row), _arr(col)
+    n = len(row) - 1
+    out = np.zeros(max(n, 1), np.int32)
+    lib().orc_color(n, row, col if len(col) else np.zeros(1, np.int32), out)
+    return out[:n]
 
 
 def bc(row, col, srcs=None):
