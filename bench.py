@@ -280,7 +280,7 @@ def color_bench(npa, ctx, args):
                    "smaller_id_edges": m_small, "colors": int(outs["depcount"].max())},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_color_count+k_color_fill+k_color_expand (host loop)", "bytes_per_launch": b_alg},
+                     "kernel": "k_color_count+k_color_fill+k_color_step (rounds captured in hipGraphs)", "bytes_per_launch": b_alg},
         "cpu_baseline": cpu, "exact": exact,
         "modes": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                   for k, v in results.items()},
