@@ -17,7 +17,7 @@ The other configurations (--config c1|c2|c4|c5) are the parity/evaluation cases 
 same path (ER 64K, R-MAT s22 per level, many sources + many uneven graphs, grid road
 graph); they print the same JSON schema.  --config bc measures betweenness centrality
 (BC_GPU) on the same frontier machinery (R-MAT s16, 1024 sources); --config color the graph
-coloring of GRAPH_COLOR_GPU (R-MAT s20).
+coloring of GRAPH_COLOR_GPU (R-MAT s20); --config pagerank PAGERANK_GPU (R-MAT s20).
 
 The CPU leg (skipped with --no-cpu) is the checker and the CPU baseline: every distance of
 every instance and mode is checked with the shortest-path certificate, a bounded sample is
@@ -290,6 +290,57 @@ def color_bench(npa, ctx, args):
     G.free()
 
 
+PR_METRIC = "PageRank GTEPS (edges x iterations) + %HBM peak @R-MAT s20, within 1e-5 of the oracle"
+
+
+def pr_bench(npa, ctx, args):
+    """--config pagerank: PAGERANK_GPU on R-MAT scale 20 (1,048,576 vertices, 16,777,216 edges, ids
+    permuted): float64 pull iterations until no rank moves by more than 1e-13 (damping 0.5, total
+    1000).  GTEPS = m x iterations / step time (the step includes building the in-edge CSR).
+    Algorithmic bytes = per iteration 12 per edge (in-edge id 4, contribution gather 8) + 48 per
+    vertex (in-edge bounds 16, rank read twice 16, contribution and new rank written 16), plus 16
+    per edge once for the in-edge CSR (out-edge ids read twice 8, in-edge id written 4, slot
+    counter 4)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    G = npa.Graph.rmat(ctx, 20, 16, wmin=1, wmax=1, seed=8)
+    row, col, _ = G.download()
+    for _ in range(args.warmup):
+        ctx.pagerank(G)
+    wall, dev = [], []
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        rank, st = ctx.pagerank(G)
+        wall.append((time.perf_counter() - t) * 1e3)
+        dev.append(st.kernel_ms)
+    ms, dev_ms, it = float(np.mean(wall)), float(np.mean(dev)), int(st.iterations)
+    b_alg = it * (12.0 * G.m + 48.0 * G.n) + 16.0 * G.m
+    achieved = b_alg / (dev_ms * 1e-3) / 1e9
+    exact = cpu = None
+    if not args.no_cpu:
+        t = time.perf_counter()
+        ref, rit = oracle.pagerank(row, col)
+        tc = time.perf_counter() - t
+        exact = bool(np.all(np.abs(rank - ref) <= 1e-5 * np.abs(ref) + 1e-9))
+        cpu = {"value": round(G.m * rit / tc / 1e9, 5), "unit": "GTEPS", "cores": 1, "kind": "port",
+               "sample": "oracle PageRank (restated iteration, float64) over the whole graph, one thread, %d "
+                         "iterations in %.1f s; the GPU ranks are within 1e-5 of it: %s" % (rit, tc, exact)}
+    line = {
+        "metric": PR_METRIC, "value": round(G.m * it / (ms * 1e-3) / 1e9, 3), "unit": "GTEPS", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "PageRank (PAGERANK_GPU), R-MAT scale 20, edge factor 16, a=.57 b=.19 c=.19, ids "
+                               "permuted; one step = one np_pagerank to convergence", "n": int(G.n), "m": int(G.m),
+                   "iterations": it, "device_ms": round(dev_ms, 3)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_pr_contrib+k_pr_pull_light+k_pr_pull_heavy (host loop)", "bytes_per_launch": b_alg},
+        "cpu_baseline": cpu, "exact": exact,
+    }
+    print(json.dumps(line), flush=True)
+    G.free()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -322,6 +373,9 @@ def main():
         return
     if args.config == "color":
         color_bench(npa, ctx, args)
+        return
+    if args.config == "pagerank":
+        pr_bench(npa, ctx, args)
         return
     t0 = time.time()
     W = build_workload(npa, ctx, args.config, args)

@@ -21,6 +21,8 @@
  *   outputBC                        common/global.cpp:377-383
  *   GRAPH_COLOR_GPU, outputColor    apps/graph-color/src/gpu/graph_color_wrapper.cu:662-699,
  *                                   common/global.cpp:393-399 (colors in graph.colorArray)
+ *   PAGERANK_GPU, outputRank,       apps/pagerank/src/gpu/pagerank_wrapper.cu:631-661,
+ *   invertGraph                     common/global.cpp:237-276, 385-391 (ranks in graph.rankArray)
  * The consolidation level is the CONSOLIDATE_LEVEL macro of the reference
  * (sssp_wrapper.cu:14-16, default 1 as in apps/sssp/src/gpu/Makefile:28; bfs_rec_wrapper.cu:
  * 20-22, default 2) given when libnpdropin.so is built, overridable at run time with the
@@ -92,5 +94,9 @@ void BC_GPU();
 
 int outputColor(FILE *file);
 void GRAPH_COLOR_GPU();
+
+int invertGraph();
+int outputRank(FILE *file);
+void PAGERANK_GPU();
 
 #endif /* NP_DROPIN_H */

@@ -319,6 +319,15 @@ int np_bc(np_ctx *ctx, np_graph *g, const int32_t *srcs, int32_t nsrc, float *bc
  * stats: iterations = rounds, kernel_ms = device time of the whole call. */
 int np_color(np_ctx *ctx, np_graph *g, int32_t method, int32_t *color_out, np_stats *stats);
 
+/* ---- PageRank: PAGERANK_GPU --------------------------------------------------------------
+ * (apps/pagerank/src/gpu/pagerank_wrapper.cu:199-244, 565-661, kernels pagerank_kernel.cu:60-107)
+ * Pull iterations over the in-edges from rank = 1000/n:
+ *   new[v] = 0.5 * sum_{p -> v} rank[p]/outdeg[p] + 0.5 * 1000/n + 0.5 * (rank of dangling)/n
+ * until no rank changes by more than 1e-13 (damping, total and epsilon of the reference), in
+ * float64; rank_out[n] receives float values (graph.rankArray, FLOAT_T = float).  max_iters = 0:
+ * no cap beyond 100000 (NP_ERR_NOT_CONVERGED past it).  stats.iterations = iterations run. */
+int np_pagerank(np_ctx *ctx, np_graph *g, int32_t max_iters, float *rank_out, np_stats *stats);
+
 /* ---- binary CSR cache -- no counterpart in the reference ----------------------------
  * A graph read or generated once is saved and read back without parsing (the reference
  * re-parses text on every run).  Layout, native little-endian: magic "NPCSR01\0", int64 n,

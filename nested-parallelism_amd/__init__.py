@@ -159,6 +159,7 @@ _SIGS = [
                                             ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
     ("np_bc", ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int32, _vp, ctypes.POINTER(Stats)]),
     ("np_color", ctypes.c_int, [_vp, _vp, ctypes.c_int32, _vp, ctypes.POINTER(Stats)]),
+    ("np_pagerank", ctypes.c_int, [_vp, _vp, ctypes.c_int32, _vp, ctypes.POINTER(Stats)]),
     ("np_csr_save", ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_int32, _i32p, _i32p,
                                    _i32p]),
     ("np_csr_load", ctypes.c_int, [ctypes.c_char_p, _i32p, ctypes.POINTER(ctypes.c_int64), _i32p,
@@ -387,6 +388,14 @@ class Context:
         out = np.empty(g.n, np.int32)
         st = Stats()
         _check(load().np_color(self.h, g.h, int(method), _ptr(out), ctypes.byref(st)))
+        return out, st
+
+    def pagerank(self, g, max_iters=0):
+        """PageRank of PAGERANK_GPU (damping 0.5, total 1000, until no change > 1e-13).
+        Returns (rank float32[n], Stats)."""
+        out = np.empty(g.n, np.float32)
+        st = Stats()
+        _check(load().np_pagerank(self.h, g.h, int(max_iters), _ptr(out), ctypes.byref(st)))
         return out, st
 
     def reached_work(self, g, d_dist_ptr, inf=INF):

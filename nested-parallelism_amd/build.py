@@ -5,7 +5,8 @@
 Outputs next to this file:
     libnpsssp.so     HIP kernels + C ABI (include/np_sssp.h)
     libnpdropin.so   the reference's C++ entry points (include/np_dropin.h) over the C ABI
-    gpu-sssp, gpu-bfs-rec, gpu-bc, gpu-graph-color   command-line drivers with the reference's flags
+    gpu-sssp, gpu-bfs-rec, gpu-bc, gpu-graph-color, gpu-pagerank   command-line drivers with the
+                            reference's flags
 """
 import os
 import subprocess
@@ -24,7 +25,7 @@ HIP_FLAGS = ARCH + ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result",
                     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-I" + INC]
 HOST_FLAGS = HIP_FLAGS  # hipcc compiles .cpp as HIP as well: same target and flags
 
-HIP_SRC = ["np_kernels.hip", "np_gen.hip", "np_bc.hip", "np_color.hip"]
+HIP_SRC = ["np_kernels.hip", "np_gen.hip", "np_bc.hip", "np_color.hip", "np_pagerank.hip"]
 HOST_SRC = ["np_api.cpp", "np_graph_io.cpp"]
 
 
@@ -67,7 +68,7 @@ def build(verbose=False):
           "-L" + HERE, "-lnpsssp", "-Wl,-rpath,$ORIGIN"])
     os.replace(dropin + ".tmp", dropin)
     for exe, flag in (("gpu-sssp", "-DNP_CLI_SSSP"), ("gpu-bfs-rec", "-DNP_CLI_BFS"), ("gpu-bc", "-DNP_CLI_BC"),
-                      ("gpu-graph-color", "-DNP_CLI_COLOR")):
+                      ("gpu-graph-color", "-DNP_CLI_COLOR"), ("gpu-pagerank", "-DNP_CLI_PR")):
         o = os.path.join(BUILD, exe + ".o")
         _run([HIPCC] + HOST_FLAGS + [flag, "-c", os.path.join(CSRC, "np_cli.cpp"), "-o", o])
         _run([HIPCC, "-o", os.path.join(HERE, exe), o, "-L" + HERE, "-lnpdropin", "-lnpsssp",

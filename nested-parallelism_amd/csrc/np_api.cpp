@@ -980,6 +980,91 @@ int np_color(np_ctx *c, np_graph *g, int32_t method, int32_t *color_out, np_stat
     return NP_OK;
 }
 
+// ---- PageRank ----------------------------------------------------------------------------
+// PAGERANK_GPU (apps/pagerank/src/gpu/pagerank_wrapper.cu:199-244, 565-661): damping 0.5, total
+// rank 1000, until no vertex changes by more than 1e-13 (pagerank_wrapper.cu:5-6, pagerank.h:15).
+struct PrBufs {
+    long long *icnt = nullptr, *irow = nullptr;
+    int *cursor = nullptr, *pcol = nullptr, *heavy = nullptr, *nheavy = nullptr;
+    double *rank = nullptr, *nrank = nullptr, *contrib = nullptr, *dsum = nullptr;
+    unsigned long long *dmax = nullptr, *h = nullptr;
+    float *out = nullptr;
+    void *temp = nullptr;
+    ~PrBufs() {
+        dfree(icnt); dfree(irow); dfree(cursor); dfree(pcol); dfree(heavy); dfree(nheavy);
+        dfree(rank); dfree(nrank); dfree(contrib); dfree(dsum); dfree(dmax); dfree(out);
+        if (temp) (void)hipFree(temp);
+        if (h) (void)hipHostFree(h);
+    }
+};
+
+int np_pagerank(np_ctx *c, np_graph *g, int32_t max_iters, float *rank_out, np_stats *stats) {
+    if (!c || !g || g->ctx != c || !rank_out || max_iters < 0) { set_error("bad arguments"); return NP_ERR_ARG; }
+    HIPCHK(hipSetDevice(c->device));
+    const int n = g->n;
+    const double damping = 0.5, total = 1000.0, eps = 1.0e-13;
+    np_stats st{};
+    st.drive = NP_DRIVE_HOSTLOOP;
+    if (n == 0) { if (stats) *stats = st; return NP_OK; }
+    PrBufs pb;
+    HIPCHK(dalloc(&pb.icnt, (size_t)n + 1));
+    HIPCHK(dalloc(&pb.irow, (size_t)n + 1));
+    HIPCHK(dalloc(&pb.cursor, (size_t)n));
+    HIPCHK(dalloc(&pb.pcol, (size_t)(g->m > 0 ? g->m : 1)));
+    HIPCHK(dalloc(&pb.heavy, (size_t)n));
+    HIPCHK(dalloc(&pb.nheavy, 1));
+    HIPCHK(dalloc(&pb.rank, (size_t)n));
+    HIPCHK(dalloc(&pb.nrank, (size_t)n));
+    HIPCHK(dalloc(&pb.contrib, (size_t)n));
+    HIPCHK(dalloc(&pb.dsum, 1));
+    HIPCHK(dalloc(&pb.dmax, 1));
+    HIPCHK(dalloc(&pb.out, (size_t)n));
+    HIPCHK(hipHostMalloc((void **)&pb.h, sizeof(unsigned long long)));
+    size_t temp_bytes = 0;
+    HIPCHK(pr_scan(nullptr, &temp_bytes, pb.icnt, pb.irow, n, c->stream));
+    HIPCHK(hipMalloc(&pb.temp, temp_bytes ? temp_bytes : 1));
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    HIPCHK(hipMemsetAsync(pb.icnt, 0, sizeof(long long) * ((size_t)n + 1), c->stream));
+    HIPCHK(hipMemsetAsync(pb.cursor, 0, sizeof(int) * (size_t)n, c->stream));
+    HIPCHK(hipMemsetAsync(pb.nheavy, 0, sizeof(int), c->stream));
+    HIPCHK(launch_pr_count(g->row, g->col, n, pb.icnt, c->stream));
+    size_t tb = temp_bytes;
+    HIPCHK(pr_scan(pb.temp, &tb, pb.icnt, pb.irow, n, c->stream));
+    HIPCHK(launch_pr_fill(g->row, g->col, n, pb.irow, pb.cursor, pb.pcol, c->stream));
+    HIPCHK(launch_pr_heavy(pb.irow, n, pb.heavy, pb.nheavy, c->stream));
+    HIPCHK(launch_pr_init(pb.rank, n, total / n, c->stream));
+    const double rw = (1.0 - damping) * total / n;
+    const int cap = max_iters > 0 ? max_iters : 100000;
+    int it = 0;
+    for (;;) {
+        if (it >= cap) { set_error("PageRank did not converge in %d iterations", cap); return NP_ERR_NOT_CONVERGED; }
+        HIPCHK(hipMemsetAsync(pb.dsum, 0, sizeof(double), c->stream));
+        HIPCHK(hipMemsetAsync(pb.dmax, 0, sizeof(unsigned long long), c->stream));
+        HIPCHK(launch_pr_iteration(g->row, pb.irow, pb.pcol, pb.heavy, pb.nheavy, pb.rank, pb.nrank, pb.contrib, n,
+                                   damping, rw, pb.dsum, pb.dmax, c->num_cus, c->stream));
+        HIPCHK(hipMemcpyAsync(pb.h, pb.dmax, sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
+        int rc = ctx_sync(c);
+        if (rc) return rc;
+        std::swap(pb.rank, pb.nrank);
+        ++it;
+        double dmax;
+        std::memcpy(&dmax, pb.h, sizeof dmax);
+        if (!(dmax > eps)) break;  // no vertex changed by more than EPSILON
+    }
+    HIPCHK(launch_pr_to_float(pb.rank, n, pb.out, c->stream));
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    HIPCHK(hipMemcpyAsync(rank_out, pb.out, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+    int rc = ctx_sync(c);
+    if (rc) return rc;
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    st.kernel_ms = ms;
+    st.iterations = it;
+    st.frontier_total = (long long)it * n;
+    if (stats) *stats = st;
+    return NP_OK;
+}
+
 int np_trace_get(np_ctx *c, np_trace_rec *recs, int32_t cap, int32_t *count) {
     if (!c || !count) { set_error("null argument"); return NP_ERR_ARG; }
     int k = std::min(c->last_iters, TRACE_CAP);

@@ -3,7 +3,8 @@
 // (apps/sssp/src/gpu/sssp.cpp:17-156), -DNP_CLI_BFS gives gpu-bfs-rec
 // (apps/bfs-rec/src/gpu/bfs_rec.cpp:17-217), -DNP_CLI_BC gives gpu-bc
 // (apps/bc/src/gpu/bc.cpp:7-140), -DNP_CLI_COLOR gives gpu-graph-color
-// (apps/graph-color/src/gpu/graph_color.cpp:7-160).  Flags: -h -v -d -f <fmt> -s <solution>
+// (apps/graph-color/src/gpu/graph_color.cpp:7-160), -DNP_CLI_PR gives gpu-pagerank
+// (apps/pagerank/src/gpu/pagerank.cpp:7-190).  Flags: -h -v -d -f <fmt> -s <solution>
 // -e <device> -i <file> (and -t, listed by the reference and ignored by its GPU binary).
 // Extra flag -o <file>: write the costs (levels, BC values) after the runs.
 #include <cstdio>
@@ -25,6 +26,8 @@ static void usage() {
     fprintf(stderr, "Usage:  gpu-bc [option]\n");
 #elif defined(NP_CLI_COLOR)
     fprintf(stderr, "Usage:  gpu-graph-color [option]\n");
+#elif defined(NP_CLI_PR)
+    fprintf(stderr, "Usage:  gpu-pagerank [option]\n");
 #else
     fprintf(stderr, "Usage:  gpu-bfs-rec [option]\n");
 #endif
@@ -62,6 +65,10 @@ static void usage() {
     fprintf(stderr, "                 7     bitmap + workload consolidation\n");
     fprintf(stderr, "                 8-11  queue working set (delayed buffers, dp)\n");
     fprintf(stderr, "                 12    queue + workload consolidation\n");
+#elif defined(NP_CLI_PR)
+    fprintf(stderr, "                 0 - pull, 1 - dual queue, 2 - shared delayed buffer,\n");
+    fprintf(stderr, "                 3 - global delayed buffer, 4 - naive dp, 5 - optimized dp,\n");
+    fprintf(stderr, "                 6 - workload consolidation\n");
 #else
     fprintf(stderr, "                 0 - GPU flat\n");
     fprintf(stderr, "                 1 - GPU naive dynamic parallelism\n");
@@ -166,7 +173,29 @@ int main(int argc, char *argv[]) {
     convertCSR();
     t1 = gettime();
     if (VERBOSE) fprintf(stderr, "AdjList to CSR:\t\t%lf\n", t1 - t0);
-#if defined(NP_CLI_COLOR)
+#if defined(NP_CLI_PR)
+    invertGraph();
+    graph.rankArray = new float[noNodeTotal > 0 ? noNodeTotal : 1];
+    for (int i = 0; i < noNodeTotal; ++i) graph.rankArray[i] = 1000.0f / noNodeTotal;  // pagerank.cpp:103-107
+    PAGERANK_GPU();  // N = 1 run (pagerank.cpp:3)
+    if (VERBOSE) {
+        fprintf(stdout, "CUDA runtime initialization:\t\t%lf\n", init_time);
+        fprintf(stdout, "CUDA H2D cudaMemcpy:\t\t%lf\n", h2d_memcpy_time);
+        fprintf(stdout, "CUDA kernel execution:\t\t%lf\n", ker_exe_time);
+        fprintf(stdout, "CUDA D2H cudaMemcpy:\t\t%lf\n", d2h_memcpy_time);
+    }
+    if (DEBUG) outputRank(stdout);
+    if (g_out) {
+        FILE *f = fopen(g_out, "w");
+        if (f) { outputRank(f); fclose(f); }
+    }
+    delete[] graph.rankArray;
+    graph.rankArray = nullptr;
+    delete[] graph.childVertexArray;
+    graph.childVertexArray = nullptr;
+    delete[] graph.rEdgeArray;
+    graph.rEdgeArray = nullptr;
+#elif defined(NP_CLI_COLOR)
     graph.colorArray = new int[noNodeTotal > 0 ? noNodeTotal : 1]();
     GRAPH_COLOR_GPU();
     if (DEBUG) outputColor(stdout);

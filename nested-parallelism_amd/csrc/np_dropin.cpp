@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <vector>
 #include <sys/time.h>
 
 #include "np_dropin.h"
@@ -175,6 +176,35 @@ int outputBC(FILE *file) {
     return 0;
 }
 
+int outputRank(FILE *file) {
+    if (!file) file = stdout;
+    fprintf(stderr, "------- Rank value ------\n");
+    for (int i = 0; i < noNodeTotal; ++i) fprintf(file, "Node %d: %0.10lf\n", i + 1, (double)graph.rankArray[i]);
+    return 0;
+}
+
+// invertGraph (common/global.cpp:237-276): the in-edge CSR in graph.childVertexArray /
+// graph.rEdgeArray, each vertex's parents in decreasing source order (the reference pushes them
+// in increasing order and pops from the back).  PAGERANK_GPU builds its own on the device;
+// this keeps the host arrays a caller of the reference may read.
+int invertGraph() {
+    delete[] graph.childVertexArray;
+    delete[] graph.rEdgeArray;
+    const int n = noNodeTotal;
+    graph.childVertexArray = new int[n + 1]();
+    graph.rEdgeArray = new int[noEdgeTotal > 0 ? noEdgeTotal : 1]();
+    for (int u = 0; u < n; ++u)
+        for (int e = graph.vertexArray[u]; e < graph.vertexArray[u + 1]; ++e) graph.childVertexArray[graph.edgeArray[e] + 1]++;
+    for (int i = 0; i < n; ++i) graph.childVertexArray[i + 1] += graph.childVertexArray[i];
+    std::vector<int> fill(graph.childVertexArray + 1, graph.childVertexArray + n + 1);  // ends, filled downwards
+    for (int u = 0; u < n; ++u)
+        for (int e = graph.vertexArray[u]; e < graph.vertexArray[u + 1]; ++e) {
+            const int t = graph.edgeArray[e];
+            graph.rEdgeArray[--fill[t]] = u;
+        }
+    return 0;
+}
+
 int outputColor(FILE *file) {
     if (!file) file = stdout;
     fprintf(stderr, "------- Color value ------\n");
@@ -290,5 +320,24 @@ void GRAPH_COLOR_GPU() {
     }
     end_time = gettime();
     fprintf(stderr, "Execution time:\t\t%lf\n", end_time - start_time);
+    np_graph_free(g);
+}
+
+// PAGERANK_GPU (apps/pagerank/src/gpu/pagerank_wrapper.cu:631-661): solutions 0-6 differ in how
+// the pull is assigned to threads and converge to the same ranks; each runs np_pagerank.  The
+// caller's main allocates graph.rankArray (pagerank.cpp:103-107).
+void PAGERANK_GPU() {
+    np_ctx *c = ctx_for_device();
+    np_graph *g = upload_graph(c);
+    np_stats st;
+    memset(&st, 0, sizeof st);
+    start_time = gettime();
+    if (config.solution >= 0 && config.solution <= 6) {
+        if (np_pagerank(c, g, 0, graph.rankArray, &st) != NP_OK) fail("PageRank");
+        if (DEBUG) fprintf(stderr, "PageRank ends in %d iterations.\n", st.iterations);
+    }
+    end_time = gettime();
+    ker_exe_time += st.kernel_ms * 1e-3;
+    d2h_memcpy_time += (end_time - start_time) - st.kernel_ms * 1e-3 > 0 ? (end_time - start_time) - st.kernel_ms * 1e-3 : 0;
     np_graph_free(g);
 }

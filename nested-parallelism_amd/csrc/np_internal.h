@@ -161,6 +161,20 @@ hipError_t launch_color_step(const int *q_in, const int *qn_in, int *q_out, int 
 hipError_t launch_color_round(const int *row, const int *col, int n, int *color, int c, int *nonstop, hipStream_t s);
 hipError_t color_scan(void *temp, size_t *temp_bytes, const long long *in, long long *out, int n, hipStream_t s);
 
+// ---- pagerank.hip: PageRank over the in-edge CSR ------------------------------------------
+hipError_t launch_pr_count(const int *row, const int *col, int n, long long *icnt, hipStream_t s);
+hipError_t launch_pr_fill(const int *row, const int *col, int n, const long long *irow, int *cursor, int *pcol,
+                          hipStream_t s);
+hipError_t launch_pr_heavy(const long long *irow, int n, int *heavy, int *nheavy, hipStream_t s);
+hipError_t launch_pr_init(double *rank, int n, double r0, hipStream_t s);
+// One iteration rank -> nrank (contrib: scratch; *dsum and *dmax must be zero on entry).
+hipError_t launch_pr_iteration(const int *row, const long long *irow, const int *pcol, const int *heavy,
+                               const int *nheavy, const double *rank, double *nrank, double *contrib, int n,
+                               double damping, double rw, double *dsum, unsigned long long *dmax, int num_cus,
+                               hipStream_t s);
+hipError_t launch_pr_to_float(const double *rank, int n, float *out, hipStream_t s);
+hipError_t pr_scan(void *temp, size_t *temp_bytes, const long long *in, long long *out, int n, hipStream_t s);
+
 // ---- gen.hip: generators and CSR construction ---------------------------------------
 struct GenOut { int *row; int *col; int *w; };
 // Builds CSR (reverse generation order per vertex) from an edge list in generation order.

@@ -30,6 +30,10 @@
  *                       the units of the algorithmic-byte count.
  *   orc_mlp_*           fp32 restatement of the scheduler MLP forward/backward; pinned
  *                       by finite differences in the tests.
+ *   orc_pagerank        the reference's PageRank iteration (pagerank_wrapper.cu:199-244,
+ *                       pagerank_kernel.cu:83-107) in double.  Pinned by the linear system of
+ *                       its fixed point solved with scipy (tests), closed forms (a directed
+ *                       cycle is uniform, a two-way star by hand) and the total rank (1000).
  *   orc_color           the colors of the reference's graph-coloring rounds
  *                       (apps/graph-color/src/gpu/graph_color_kernel.cu:14-35):
  *                       color[v] = 1 + max color of the out-neighbours with a smaller id.
@@ -492,4 +496,41 @@ void orc_color(int n, const int *row, const int *col, int *color) {
             if (col[e] < v && color[col[e]] > c) c = color[col[e]];
         color[v] = c + 1;
     }
+}
+
+/* PageRank as PAGERANK_GPU iterates it: from rank = total/n, each iteration
+ *   new[v] = damping * sum over edges p -> v of rank[p]/outdeg[p]
+ *            + (1 - damping) * total / n + damping * (sum of dangling ranks) / n,
+ * until no vertex changes by more than eps (or max_iters).  Sums are pushed along out-edges in
+ * vertex order (the order only moves the result by rounding).  Returns the iterations run. */
+int orc_pagerank(int n, const int *row, const int *col, double damping, double total, double eps,
+                 int max_iters, double *rank) {
+    double *nr = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    if (!nr) return -1;
+    for (int v = 0; v < n; ++v) rank[v] = total / n;
+    int it = 0;
+    while (it < max_iters) {
+        double dang = 0.0;
+        for (int v = 0; v < n; ++v)
+            if (row[v + 1] == row[v]) dang += rank[v];
+        const double add = (1.0 - damping) * total / n + damping * dang / n;
+        for (int v = 0; v < n; ++v) nr[v] = 0.0;
+        for (int u = 0; u < n; ++u) {
+            const int d = row[u + 1] - row[u];
+            if (d == 0) continue;
+            const double c = rank[u] / d;
+            for (int e = row[u]; e < row[u + 1]; ++e) nr[col[e]] += c;
+        }
+        double mx = 0.0;
+        for (int v = 0; v < n; ++v) {
+            const double r = damping * nr[v] + add;
+            const double dd = fabs(r - rank[v]);
+            if (dd > mx) mx = dd;
+            rank[v] = r;
+        }
+        ++it;
+        if (!(mx > eps)) break;
+    }
+    free(nr);
+    return it;
 }

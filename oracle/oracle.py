@@ -57,6 +57,9 @@ def lib():
                                        ctypes.POINTER(ll), ctypes.POINTER(ll)]
         L.orc_mlp_nparam.restype = ctypes.c_int
         L.orc_mlp_forward.restype = None
+        L.orc_pagerank.argtypes = [ctypes.c_int, _i32p, _i32p, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                   ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+        L.orc_pagerank.restype = ctypes.c_int
         L.orc_color.argtypes = [ctypes.c_int, _i32p, _i32p, _i32p]
         L.orc_color.restype = None
         L.orc_bc.argtypes = [ctypes.c_int, _i32p, _i32p, _i32p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
@@ -160,6 +163,18 @@ def reached_work(row, dist, inf=INF):
 
 
 MLP_NPARAM = 10 * 32 + 32 + 32 * 32 + 32 + 5 * 32 + 5
+
+
+def pagerank(row, col, damping=0.5, total=1000.0, eps=1e-13, max_iters=100000):
+    """PageRank as PAGERANK_GPU iterates it (float64[n], iterations)."""
+    row, col = _arr(row), _arr(col)
+    n = len(row) - 1
+    out = np.zeros(max(n, 1), np.float64)
+    it = lib().orc_pagerank(n, row, col if len(col) else np.zeros(1, np.int32), damping, total, eps, max_iters,
+                            out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    if it < 0:
+        raise ValueError("orc_pagerank failed")
+    return out[:n], it
 
 
 def color(row, col):
