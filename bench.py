@@ -17,7 +17,8 @@ The other configurations (--config c1|c2|c4|c5) are the parity/evaluation cases 
 same path (ER 64K, R-MAT s22 per level, many sources + many uneven graphs, grid road
 graph); they print the same JSON schema.  --config bc measures betweenness centrality
 (BC_GPU) on the same frontier machinery (R-MAT s16, 1024 sources); --config color the graph
-coloring of GRAPH_COLOR_GPU (R-MAT s20); --config pagerank PAGERANK_GPU (R-MAT s20).
+coloring of GRAPH_COLOR_GPU (R-MAT s20); --config pagerank PAGERANK_GPU (R-MAT s20); --config
+spmv SpMV_GPU (R-MAT s22).
 
 The CPU leg (skipped with --no-cpu) is the checker and the CPU baseline: every distance of
 every instance and mode is checked with the shortest-path certificate, a bounded sample is
@@ -341,6 +342,55 @@ def pr_bench(npa, ctx, args):
     G.free()
 
 
+SPMV_METRIC = "SpMV GFLOP/s + %HBM peak @R-MAT s22 (float64), exact"
+
+
+def spmv_bench(npa, ctx, args):
+    """--config spmv: SpMV_GPU on R-MAT scale 22 (4,194,304 rows, 67,108,864 entries, ids permuted)
+    with the reference's values (data[i] = i % 13 + 0.5, x[i] = i % 17 + 0.5, float64: every sum is
+    exact, so the check is equality).  GFLOP/s = 2 m / the product's device time (data, x and the
+    CSR resident in HBM; the host copies np_spmv also makes are reported apart).  Algorithmic bytes
+    = 20 per entry (column 4, value 8, x gather 8) + 16 per row (row bounds 8, y 8)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    G = npa.Graph.rmat(ctx, 22, 16, wmin=1, wmax=1, seed=9)
+    row, col, _ = G.download()
+    data, x = oracle.spmv_reference_values(G.n, G.m)
+    for _ in range(args.warmup):
+        ctx.spmv(G, data, x)
+    wall, dev = [], []
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        y, st = ctx.spmv(G, data, x)
+        wall.append((time.perf_counter() - t) * 1e3)
+        dev.append(st.kernel_ms)
+    dev_ms, wall_ms = float(np.mean(dev)), float(np.mean(wall))
+    b_alg = 20.0 * G.m + 16.0 * G.n
+    achieved = b_alg / (dev_ms * 1e-3) / 1e9
+    exact = cpu = None
+    if not args.no_cpu:
+        t = time.perf_counter()
+        ref = oracle.spmv(row, col, data, x)
+        tc = time.perf_counter() - t
+        exact = bool(np.array_equal(y, ref))
+        cpu = {"value": round(2.0 * G.m / tc / 1e9, 4), "unit": "GFLOP/s", "cores": 1, "kind": "port",
+               "sample": "oracle CSR product over the whole matrix, one thread, %.2f s; GPU result equal: %s" % (tc, exact)}
+    line = {
+        "metric": SPMV_METRIC, "value": round(2.0 * G.m / (dev_ms * 1e-3) / 1e9, 2), "unit": "GFLOP/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dev_ms, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "SpMV (SpMV_GPU), R-MAT scale 22, edge factor 16, a=.57 b=.19 c=.19, ids permuted, "
+                               "the reference's values; one step = one product", "n": int(G.n), "m": int(G.m),
+                   "wall_ms_with_host_copies": round(wall_ms, 3)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_spmv_heavy_list+k_spmv_light+k_spmv_heavy", "bytes_per_launch": b_alg},
+        "cpu_baseline": cpu, "exact": exact,
+    }
+    print(json.dumps(line), flush=True)
+    G.free()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -376,6 +426,9 @@ def main():
         return
     if args.config == "pagerank":
         pr_bench(npa, ctx, args)
+        return
+    if args.config == "spmv":
+        spmv_bench(npa, ctx, args)
         return
     t0 = time.time()
     W = build_workload(npa, ctx, args.config, args)

@@ -23,6 +23,10 @@
  *                                   common/global.cpp:393-399 (colors in graph.colorArray)
  *   PAGERANK_GPU, outputRank,       apps/pagerank/src/gpu/pagerank_wrapper.cu:631-661,
  *   invertGraph                     common/global.cpp:237-276, 385-391 (ranks in graph.rankArray)
+ *   SpMV_GPU                        apps/SpMV/src/gpu/SpMV_wrapper.cu:97-113, 314-345: fills the
+ *                                   caller's double *data, *x, *y (defined by its main, as in
+ *                                   apps/SpMV/src/gpu/SpMV.cpp:9-11; referenced weakly here, so
+ *                                   programs without them still link) and computes y = A x
  * The consolidation level is the CONSOLIDATE_LEVEL macro of the reference
  * (sssp_wrapper.cu:14-16, default 1 as in apps/sssp/src/gpu/Makefile:28; bfs_rec_wrapper.cu:
  * 20-22, default 2) given when libnpdropin.so is built, overridable at run time with the
@@ -98,5 +102,7 @@ void GRAPH_COLOR_GPU();
 int invertGraph();
 int outputRank(FILE *file);
 void PAGERANK_GPU();
+
+void SpMV_GPU();
 
 #endif /* NP_DROPIN_H */

@@ -328,6 +328,11 @@ int np_color(np_ctx *ctx, np_graph *g, int32_t method, int32_t *color_out, np_st
  * no cap beyond 100000 (NP_ERR_NOT_CONVERGED past it).  stats.iterations = iterations run. */
 int np_pagerank(np_ctx *ctx, np_graph *g, int32_t max_iters, float *rank_out, np_stats *stats);
 
+/* ---- SpMV: SpMV_GPU (apps/SpMV/src/gpu/SpMV_wrapper.cu:314-345, SpMV_kernel.cu:77-92) ------
+ * y[v] = sum over the graph's row v of data[e] * x[col[e]] (float64, the reference's FLOAT_T);
+ * host arrays data[m], x[n], y[n].  kernel_ms covers the product only (not the copies). */
+int np_spmv(np_ctx *ctx, np_graph *g, const double *data, const double *x, double *y, np_stats *stats);
+
 /* ---- binary CSR cache -- no counterpart in the reference ----------------------------
  * A graph read or generated once is saved and read back without parsing (the reference
  * re-parses text on every run).  Layout, native little-endian: magic "NPCSR01\0", int64 n,

@@ -160,6 +160,7 @@ _SIGS = [
     ("np_bc", ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int32, _vp, ctypes.POINTER(Stats)]),
     ("np_color", ctypes.c_int, [_vp, _vp, ctypes.c_int32, _vp, ctypes.POINTER(Stats)]),
     ("np_pagerank", ctypes.c_int, [_vp, _vp, ctypes.c_int32, _vp, ctypes.POINTER(Stats)]),
+    ("np_spmv", ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.POINTER(Stats)]),
     ("np_csr_save", ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_int32, _i32p, _i32p,
                                    _i32p]),
     ("np_csr_load", ctypes.c_int, [ctypes.c_char_p, _i32p, ctypes.POINTER(ctypes.c_int64), _i32p,
@@ -397,6 +398,17 @@ class Context:
         st = Stats()
         _check(load().np_pagerank(self.h, g.h, int(max_iters), _ptr(out), ctypes.byref(st)))
         return out, st
+
+    def spmv(self, g, data, x):
+        """SpMV_GPU: y = A x over g's CSR with edge values data (float64).  Returns (y, Stats)."""
+        data = np.ascontiguousarray(data, np.float64)
+        x = np.ascontiguousarray(x, np.float64)
+        if len(data) != g.m or len(x) != g.n:
+            raise ValueError("data must have m entries and x n")
+        y = np.empty(g.n, np.float64)
+        st = Stats()
+        _check(load().np_spmv(self.h, g.h, _ptr(data), _ptr(x), _ptr(y), ctypes.byref(st)))
+        return y, st
 
     def reached_work(self, g, d_dist_ptr, inf=INF):
         a, b = ctypes.c_int64(), ctypes.c_int64()

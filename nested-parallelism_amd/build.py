@@ -5,7 +5,7 @@
 Outputs next to this file:
     libnpsssp.so     HIP kernels + C ABI (include/np_sssp.h)
     libnpdropin.so   the reference's C++ entry points (include/np_dropin.h) over the C ABI
-    gpu-sssp, gpu-bfs-rec, gpu-bc, gpu-graph-color, gpu-pagerank   command-line drivers with the
+    gpu-sssp, gpu-bfs-rec, gpu-bc, gpu-graph-color, gpu-pagerank, gpu-spmv   command-line drivers with the
                             reference's flags
 """
 import os
@@ -68,7 +68,8 @@ def build(verbose=False):
           "-L" + HERE, "-lnpsssp", "-Wl,-rpath,$ORIGIN"])
     os.replace(dropin + ".tmp", dropin)
     for exe, flag in (("gpu-sssp", "-DNP_CLI_SSSP"), ("gpu-bfs-rec", "-DNP_CLI_BFS"), ("gpu-bc", "-DNP_CLI_BC"),
-                      ("gpu-graph-color", "-DNP_CLI_COLOR"), ("gpu-pagerank", "-DNP_CLI_PR")):
+                      ("gpu-graph-color", "-DNP_CLI_COLOR"), ("gpu-pagerank", "-DNP_CLI_PR"),
+                      ("gpu-spmv", "-DNP_CLI_SPMV")):
         o = os.path.join(BUILD, exe + ".o")
         _run([HIPCC] + HOST_FLAGS + [flag, "-c", os.path.join(CSRC, "np_cli.cpp"), "-o", o])
         _run([HIPCC, "-o", os.path.join(HERE, exe), o, "-L" + HERE, "-lnpdropin", "-lnpsssp",

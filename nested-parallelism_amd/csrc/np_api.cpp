@@ -1065,6 +1065,44 @@ int np_pagerank(np_ctx *c, np_graph *g, int32_t max_iters, float *rank_out, np_s
     return NP_OK;
 }
 
+// ---- SpMV: SpMV_GPU (apps/SpMV/src/gpu/SpMV_wrapper.cu:314-345) --------------------------
+int np_spmv(np_ctx *c, np_graph *g, const double *data, const double *x, double *y, np_stats *stats) {
+    if (!c || !g || g->ctx != c || !x || !y || (g->m > 0 && !data)) { set_error("bad arguments"); return NP_ERR_ARG; }
+    HIPCHK(hipSetDevice(c->device));
+    const int n = g->n;
+    const long long m = g->m;
+    np_stats st{};
+    st.drive = NP_DRIVE_HOSTLOOP;
+    if (n == 0) { if (stats) *stats = st; return NP_OK; }
+    double *d_data = nullptr, *d_x = nullptr, *d_y = nullptr;
+    int *heavy = nullptr, *nheavy = nullptr;
+    struct Free {
+        double **a, **b, **c; int **h, **nh;
+        ~Free() { dfree(*a); dfree(*b); dfree(*c); dfree(*h); dfree(*nh); }
+    } fr{&d_data, &d_x, &d_y, &heavy, &nheavy};
+    HIPCHK(dalloc(&d_data, (size_t)(m > 0 ? m : 1)));
+    HIPCHK(dalloc(&d_x, (size_t)n));
+    HIPCHK(dalloc(&d_y, (size_t)n));
+    HIPCHK(dalloc(&heavy, (size_t)n));
+    HIPCHK(dalloc(&nheavy, 1));
+    if (m > 0) HIPCHK(hipMemcpyAsync(d_data, data, sizeof(double) * (size_t)m, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(d_x, x, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemsetAsync(nheavy, 0, sizeof(int), c->stream));
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    HIPCHK(launch_spmv_heavy_list(g->row, n, heavy, nheavy, c->stream));
+    HIPCHK(launch_spmv(g->row, g->col, d_data, d_x, d_y, n, heavy, nheavy, c->num_cus, c->stream));
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    HIPCHK(hipMemcpyAsync(y, d_y, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+    int rc = ctx_sync(c);
+    if (rc) return rc;
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    st.kernel_ms = ms;
+    st.iterations = 1;
+    if (stats) *stats = st;
+    return NP_OK;
+}
+
 int np_trace_get(np_ctx *c, np_trace_rec *recs, int32_t cap, int32_t *count) {
     if (!c || !count) { set_error("null argument"); return NP_ERR_ARG; }
     int k = std::min(c->last_iters, TRACE_CAP);

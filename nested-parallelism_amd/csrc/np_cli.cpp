@@ -4,7 +4,8 @@
 // (apps/bfs-rec/src/gpu/bfs_rec.cpp:17-217), -DNP_CLI_BC gives gpu-bc
 // (apps/bc/src/gpu/bc.cpp:7-140), -DNP_CLI_COLOR gives gpu-graph-color
 // (apps/graph-color/src/gpu/graph_color.cpp:7-160), -DNP_CLI_PR gives gpu-pagerank
-// (apps/pagerank/src/gpu/pagerank.cpp:7-190).  Flags: -h -v -d -f <fmt> -s <solution>
+// (apps/pagerank/src/gpu/pagerank.cpp:7-190), -DNP_CLI_SPMV gives gpu-spmv
+// (apps/SpMV/src/gpu/SpMV.cpp:7-165).  Flags: -h -v -d -f <fmt> -s <solution>
 // -e <device> -i <file> (and -t, listed by the reference and ignored by its GPU binary).
 // Extra flag -o <file>: write the costs (levels, BC values) after the runs.
 #include <cstdio>
@@ -18,6 +19,10 @@
 
 static const char *g_out = nullptr;
 
+#ifdef NP_CLI_SPMV
+double *data = nullptr, *x = nullptr, *y = nullptr;  // apps/SpMV/src/gpu/SpMV.cpp:9-11
+#endif
+
 static void usage() {
     fprintf(stderr, "\n");
 #if defined(NP_CLI_SSSP)
@@ -28,6 +33,8 @@ static void usage() {
     fprintf(stderr, "Usage:  gpu-graph-color [option]\n");
 #elif defined(NP_CLI_PR)
     fprintf(stderr, "Usage:  gpu-pagerank [option]\n");
+#elif defined(NP_CLI_SPMV)
+    fprintf(stderr, "Usage:  gpu-spmv [option]\n");
 #else
     fprintf(stderr, "Usage:  gpu-bfs-rec [option]\n");
 #endif
@@ -67,6 +74,10 @@ static void usage() {
     fprintf(stderr, "                 12    queue + workload consolidation\n");
 #elif defined(NP_CLI_PR)
     fprintf(stderr, "                 0 - pull, 1 - dual queue, 2 - shared delayed buffer,\n");
+    fprintf(stderr, "                 3 - global delayed buffer, 4 - naive dp, 5 - optimized dp,\n");
+    fprintf(stderr, "                 6 - workload consolidation\n");
+#elif defined(NP_CLI_SPMV)
+    fprintf(stderr, "                 0 - thread per row, 1 - dual queue, 2 - shared delayed buffer,\n");
     fprintf(stderr, "                 3 - global delayed buffer, 4 - naive dp, 5 - optimized dp,\n");
     fprintf(stderr, "                 6 - workload consolidation\n");
 #else
@@ -173,7 +184,29 @@ int main(int argc, char *argv[]) {
     convertCSR();
     t1 = gettime();
     if (VERBOSE) fprintf(stderr, "AdjList to CSR:\t\t%lf\n", t1 - t0);
-#if defined(NP_CLI_PR)
+#if defined(NP_CLI_SPMV)
+    SpMV_GPU();  // N = 1 run (SpMV.cpp:3)
+    if (VERBOSE) {
+        fprintf(stdout, "CUDA runtime initialization:\t\t%lf\n", init_time);
+        fprintf(stdout, "CUDA H2D cudaMemcpy:\t\t%lf\n", h2d_memcpy_time);
+        fprintf(stdout, "CUDA kernel execution:\t\t%lf\n", ker_exe_time);
+        fprintf(stdout, "CUDA D2H cudaMemcpy:\t\t%lf\n", d2h_memcpy_time);
+    }
+    if (DEBUG)
+        for (int i = 0; i < noNodeTotal; ++i) {
+            fprintf(stderr, "x[%d] = %f\n", i, x[i]);
+            fprintf(stderr, "y[%d] = %f\n", i, y[i]);
+        }
+    if (g_out) {
+        FILE *f = fopen(g_out, "w");
+        if (f) {
+            for (int i = 0; i < noNodeTotal; ++i) fprintf(f, "y[%d] = %.17g\n", i, y[i]);
+            fclose(f);
+        }
+    }
+    delete[] data; delete[] x; delete[] y;
+    data = x = y = nullptr;
+#elif defined(NP_CLI_PR)
     invertGraph();
     graph.rankArray = new float[noNodeTotal > 0 ? noNodeTotal : 1];
     for (int i = 0; i < noNodeTotal; ++i) graph.rankArray[i] = 1000.0f / noNodeTotal;  // pagerank.cpp:103-107

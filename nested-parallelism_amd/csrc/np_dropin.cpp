@@ -341,3 +341,34 @@ void PAGERANK_GPU() {
     d2h_memcpy_time += (end_time - start_time) - st.kernel_ms * 1e-3 > 0 ? (end_time - start_time) - st.kernel_ms * 1e-3 : 0;
     np_graph_free(g);
 }
+
+// SpMV_GPU (apps/SpMV/src/gpu/SpMV_wrapper.cu:97-113, 314-345): allocates and fills the caller's
+// data (i % 13 + 0.5 per edge) and x (i % 17 + 0.5 per vertex) as the reference's prepare_gpu
+// does, then y = A x over the global CSR (solutions 0-6 all compute this product).  The arrays
+// are the main program's globals; they are weak references here, so a program that defines
+// none of them (the other drivers) still loads, and SpMV_GPU then stops with an error.
+extern double *data __attribute__((weak));
+extern double *x __attribute__((weak));
+extern double *y __attribute__((weak));
+
+void SpMV_GPU() {
+    if (!&data || !&x || !&y) fail("SpMV_GPU needs the globals double *data, *x, *y of its main program");
+    np_ctx *c = ctx_for_device();
+    np_graph *g = upload_graph(c);
+    delete[] data; delete[] x; delete[] y;
+    data = new double[noEdgeTotal > 0 ? noEdgeTotal : 1];
+    x = new double[noNodeTotal > 0 ? noNodeTotal : 1];
+    y = new double[noNodeTotal > 0 ? noNodeTotal : 1]();
+    for (int i = 0; i < noEdgeTotal; ++i) data[i] = i % 13 + 0.5;
+    for (int i = 0; i < noNodeTotal; ++i) x[i] = i % 17 + 0.5;
+    np_stats st;
+    memset(&st, 0, sizeof st);
+    start_time = gettime();
+    if (config.solution >= 0 && config.solution <= 6) {
+        if (np_spmv(c, g, data, x, y, &st) != NP_OK) fail("SpMV");
+    }
+    end_time = gettime();
+    ker_exe_time += st.kernel_ms * 1e-3;
+    d2h_memcpy_time += (end_time - start_time) - st.kernel_ms * 1e-3 > 0 ? (end_time - start_time) - st.kernel_ms * 1e-3 : 0;
+    np_graph_free(g);
+}

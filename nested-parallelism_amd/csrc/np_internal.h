@@ -173,6 +173,10 @@ hipError_t launch_pr_iteration(const int *row, const long long *irow, const int 
                                double damping, double rw, double *dsum, unsigned long long *dmax, int num_cus,
                                hipStream_t s);
 hipError_t launch_pr_to_float(const double *rank, int n, float *out, hipStream_t s);
+// SpMV y = A x over the graph's CSR (rows of >= 32 entries listed in heavy[*nheavy]).
+hipError_t launch_spmv_heavy_list(const int *row, int n, int *heavy, int *nheavy, hipStream_t s);
+hipError_t launch_spmv(const int *row, const int *col, const double *data, const double *x, double *y, int n,
+                       const int *heavy, const int *nheavy, int num_cus, hipStream_t s);
 hipError_t pr_scan(void *temp, size_t *temp_bytes, const long long *in, long long *out, int n, hipStream_t s);
 
 // ---- gen.hip: generators and CSR construction ---------------------------------------

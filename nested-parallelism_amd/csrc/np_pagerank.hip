@@ -11,10 +11,12 @@
 // one wavefront per vertex); each iteration is two kernels: contributions rank/outdeg with the
 // dangling sum (workgroup reduction, one float64 atomic per workgroup), then the pull with the
 // largest change (the same reduction, an integer atomicMax on the non-negative float64 bit
-// pattern).  The pull splits vertices by in-degree: fewer than 32 in-edges, one thread per
-// vertex; more, one wavefront per vertex with a wave reduction (heavy list built once).  All in
+// pattern).  The pull splits vertices by in-degree: fewer than 32 in-edges, 8 lanes per vertex
+// (consecutive in-edges, coalesced); more, one wavefront per vertex (heavy list built once); both
+// reduce with shuffles.  All in
 // float64, so the reference's EPSILON is reachable (float ranks near 1000/n cannot move by
-// less than an ulp ~1e-9); ranks are returned as float32, the reference's FLOAT_T.
+// less than an ulp ~1e-9); ranks are returned as float32, the reference's FLOAT_T.  The file also
+// holds SpMV (apps/SpMV), the same row-split product without PageRank's terms.
 #include <hip/hip_runtime.h>
 #include <cstring>
 #include <rocprim/device/device_scan.hpp>
@@ -112,18 +114,26 @@ __global__ __launch_bounds__(256) void k_pr_contrib(const int *row, const double
     pr_block_add(dang, dsum);
 }
 
-// light vertices (fewer than PR_HEAVY in-edges): one thread each.  base_add = the random-walk and
-// dangling terms of this iteration; *dmax receives the largest |new - old|.
+// light vertices (fewer than PR_HEAVY in-edges): PR_GROUP lanes each, reading consecutive
+// in-edges (coalesced), summed with shuffles.  base_add = the random-walk and dangling terms of
+// this iteration; *dmax receives the largest |new - old|.
+constexpr int PR_GROUP = 8;
 __global__ __launch_bounds__(256) void k_pr_pull_light(const long long *irow, const int *pcol, const double *contrib,
                                                         const double *rank, double *nrank, int n, double damping,
                                                         const double *dsum, double rw, unsigned long long *dmax) {
     const double base_add = rw + damping * (*dsum) / (double)n;
+    const int sub = threadIdx.x & (PR_GROUP - 1);
+    const long long groups = (long long)gridDim.x * (blockDim.x / PR_GROUP);
     double mx = 0.0;
-    for (long long base = (long long)blockIdx.x * blockDim.x; base < n; base += (long long)gridDim.x * blockDim.x) {
-        const long long v = base + threadIdx.x;
-        if (v < n && irow[v + 1] - irow[v] < PR_HEAVY) {
-            double s = 0.0;
-            for (long long i = irow[v]; i < irow[v + 1]; ++i) s += contrib[pcol[i]];
+    // every lane of a wave runs the same number of trips (the bound is rounded to whole waves)
+    const long long nv = ((long long)n + 63) / 64 * 64;
+    for (long long v = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / PR_GROUP; v < nv; v += groups) {
+        const bool light = v < n && irow[v + 1] - irow[v] < PR_HEAVY;
+        double s = 0.0;
+        if (light)
+            for (long long i = irow[v] + sub; i < irow[v + 1]; i += PR_GROUP) s += contrib[pcol[i]];
+        for (int o = PR_GROUP / 2; o > 0; o >>= 1) s += __shfl_xor(s, o);
+        if (light && sub == 0) {
             const double r = damping * s + base_add;
             nrank[v] = r;
             const double dd = fabs(r - rank[v]);
@@ -158,6 +168,51 @@ __global__ __launch_bounds__(256) void k_pr_pull_heavy(const int *heavy, const i
     pr_block_max(mx, dmax);
 }
 
+// ---- SpMV (apps/SpMV/src/gpu/SpMV_kernel.cu:77-92, csr_spmv_kernel): y[v] = sum over the
+// row's entries of data[e] * x[col[e]], the same light/heavy split as the pull (rows with fewer
+// than PR_HEAVY entries PR_GROUP lanes each, longer rows one wavefront each).
+__global__ void k_spmv_heavy_list(const int *row, int n, int *heavy, int *nheavy) {
+    for (long long base = (long long)blockIdx.x * blockDim.x; base < n; base += (long long)gridDim.x * blockDim.x) {
+        const long long v = base + threadIdx.x;
+        const bool h = v < n && row[v + 1] - row[v] >= PR_HEAVY;
+        const unsigned long long want = __ballot(h);
+        if (want == 0) continue;
+        const int lane = pr_lane();
+        const int leader = __ffsll((long long)want) - 1;
+        int b = 0;
+        if (lane == leader) b = atomicAdd(nheavy, __popcll(want));
+        b = __shfl(b, leader);
+        if (h) heavy[b + __popcll(want & ((1ull << lane) - 1))] = (int)v;
+    }
+}
+__global__ __launch_bounds__(256) void k_spmv_light(const int *row, const int *col, const double *data,
+                                                     const double *x, double *y, int n) {
+    const int sub = threadIdx.x & (PR_GROUP - 1);
+    const long long groups = (long long)gridDim.x * (blockDim.x / PR_GROUP);
+    const long long nv = ((long long)n + 63) / 64 * 64;  // whole waves: same trips in every lane
+    for (long long v = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / PR_GROUP; v < nv; v += groups) {
+        const bool light = v < n && row[v + 1] - row[v] < PR_HEAVY;
+        double s = 0.0;
+        if (light)
+            for (int e = row[v] + sub; e < row[v + 1]; e += PR_GROUP) s += data[e] * x[col[e]];
+        for (int o = PR_GROUP / 2; o > 0; o >>= 1) s += __shfl_xor(s, o);
+        if (light && sub == 0) y[v] = s;
+    }
+}
+__global__ __launch_bounds__(256) void k_spmv_heavy(const int *heavy, const int *nheavy, const int *row, const int *col,
+                                                     const double *data, const double *x, double *y) {
+    const int lane = pr_lane();
+    const int nh = *nheavy;
+    const long long nw = (long long)gridDim.x * (blockDim.x >> 6);
+    for (long long o = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); o < nh; o += nw) {
+        const int v = heavy[o];
+        double s = 0.0;
+        for (int e = row[v] + lane; e < row[v + 1]; e += 64) s += data[e] * x[col[e]];
+        for (int k = 32; k > 0; k >>= 1) s += __shfl_xor(s, k);
+        if (lane == 0) y[v] = s;
+    }
+}
+
 __global__ void k_pr_to_float(const double *rank, int n, float *out) {
     for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (long long)gridDim.x * blockDim.x)
         out[v] = (float)rank[v];
@@ -186,10 +241,22 @@ hipError_t launch_pr_iteration(const int *row, const long long *irow, const int 
                                double damping, double rw, double *dsum, unsigned long long *dmax, int num_cus,
                                hipStream_t s) {
     hipLaunchKernelGGL(k_pr_contrib, dim3(pr_grid(n, num_cus * 4)), dim3(256), 0, s, row, rank, n, contrib, dsum);
-    hipLaunchKernelGGL(k_pr_pull_light, dim3(pr_grid(n, num_cus * 8)), dim3(256), 0, s, irow, pcol, contrib, rank,
+    hipLaunchKernelGGL(k_pr_pull_light, dim3(pr_grid((long long)n * PR_GROUP, num_cus * 16)), dim3(256), 0, s, irow,
+                       pcol, contrib, rank,
                        nrank, n, damping, dsum, rw, dmax);
     hipLaunchKernelGGL(k_pr_pull_heavy, dim3(num_cus * 4), dim3(256), 0, s, heavy, nheavy, irow, pcol, contrib, rank,
                        nrank, n, damping, dsum, rw, dmax);
+    return hipGetLastError();
+}
+hipError_t launch_spmv_heavy_list(const int *row, int n, int *heavy, int *nheavy, hipStream_t s) {
+    hipLaunchKernelGGL(k_spmv_heavy_list, dim3(pr_grid(n, 4096)), dim3(256), 0, s, row, n, heavy, nheavy);
+    return hipGetLastError();
+}
+hipError_t launch_spmv(const int *row, const int *col, const double *data, const double *x, double *y, int n,
+                       const int *heavy, const int *nheavy, int num_cus, hipStream_t s) {
+    hipLaunchKernelGGL(k_spmv_light, dim3(pr_grid((long long)n * PR_GROUP, num_cus * 16)), dim3(256), 0, s, row, col,
+                       data, x, y, n);
+    hipLaunchKernelGGL(k_spmv_heavy, dim3(num_cus * 4), dim3(256), 0, s, heavy, nheavy, row, col, data, x, y);
     return hipGetLastError();
 }
 hipError_t launch_pr_to_float(const double *rank, int n, float *out, hipStream_t s) {

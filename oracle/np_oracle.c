@@ -34,6 +34,8 @@
  *                       pagerank_kernel.cu:83-107) in double.  Pinned by the linear system of
  *                       its fixed point solved with scipy (tests), closed forms (a directed
  *                       cycle is uniform, a two-way star by hand) and the total rank (1000).
+ *   orc_spmv            y = A x over the CSR (apps/SpMV/src/gpu/SpMV_kernel.cu:77-92) in
+ *                       double.  Pinned by scipy's CSR product and closed forms (tests).
  *   orc_color           the colors of the reference's graph-coloring rounds
  *                       (apps/graph-color/src/gpu/graph_color_kernel.cu:14-35):
  *                       color[v] = 1 + max color of the out-neighbours with a smaller id.
@@ -533,4 +535,13 @@ int orc_pagerank(int n, const int *row, const int *col, double damping, double t
     }
     free(nr);
     return it;
+}
+
+/* y[v] = sum over row v of data[e] * x[col[e]] (csr_spmv_kernel, SpMV_kernel.cu:77-92). */
+void orc_spmv(int n, const int *row, const int *col, const double *data, const double *x, double *y) {
+    for (int v = 0; v < n; ++v) {
+        double s = 0.0;
+        for (int e = row[v]; e < row[v + 1]; ++e) s += data[e] * x[col[e]];
+        y[v] = s;
+    }
 }

@@ -60,6 +60,9 @@ def lib():
         L.orc_pagerank.argtypes = [ctypes.c_int, _i32p, _i32p, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                    ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
         L.orc_pagerank.restype = ctypes.c_int
+        L.orc_spmv.argtypes = [ctypes.c_int, _i32p, _i32p, ctypes.POINTER(ctypes.c_double),
+                               ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+        L.orc_spmv.restype = None
         L.orc_color.argtypes = [ctypes.c_int, _i32p, _i32p, _i32p]
         L.orc_color.restype = None
         L.orc_bc.argtypes = [ctypes.c_int, _i32p, _i32p, _i32p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
@@ -175,6 +178,24 @@ def pagerank(row, col, damping=0.5, total=1000.0, eps=1e-13, max_iters=100000):
     if it < 0:
         raise ValueError("orc_pagerank failed")
     return out[:n], it
+
+
+def spmv(row, col, data, x):
+    """y = A x over the CSR (float64)."""
+    row, col = _arr(row), _arr(col)
+    n = len(row) - 1
+    d = np.ascontiguousarray(data, np.float64)
+    xx = np.ascontiguousarray(x, np.float64)
+    y = np.zeros(max(n, 1), np.float64)
+    dp = ctypes.POINTER(ctypes.c_double)
+    lib().orc_spmv(n, row, col if len(col) else np.zeros(1, np.int32),
+                   (d if len(d) else np.zeros(1)).ctypes.data_as(dp), xx.ctypes.data_as(dp), y.ctypes.data_as(dp))
+    return y[:n]
+
+
+def spmv_reference_values(n, m):
+    """The reference's inputs (SpMV_wrapper.cu:104-113): data[i] = i % 13 + 0.5, x[i] = i % 17 + 0.5."""
+    return np.arange(m) % 13 + 0.5, np.arange(n) % 17 + 0.5
 
 
 def color(row, col):

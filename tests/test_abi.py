@@ -40,13 +40,13 @@ def test_dropin_exports_reference_names():
             "_Z5clearv", "_Z7gettimev", "_Z10gettime_msv", "graph", "noNodeTotal", "noEdgeTotal", "source",
             "fp", "VERBOSE", "DEBUG", "config", "ker_exe_time", "h2d_memcpy_time", "d2h_memcpy_time",
             "_Z6BC_GPUv", "_Z8outputBCP8_IO_FILE", "bc", "_Z15GRAPH_COLOR_GPUv", "_Z11outputColorP8_IO_FILE",
-            "_Z12PAGERANK_GPUv", "_Z10outputRankP8_IO_FILE", "_Z11invertGraphv"]
+            "_Z12PAGERANK_GPUv", "_Z10outputRankP8_IO_FILE", "_Z11invertGraphv", "_Z8SpMV_GPUv"]
     missing = [n for n in need if n not in syms]
     assert not missing, missing
 
 
 def test_cli_binaries_built():
-    for exe in ("gpu-sssp", "gpu-bfs-rec", "gpu-bc", "gpu-graph-color", "gpu-pagerank"):
+    for exe in ("gpu-sssp", "gpu-bfs-rec", "gpu-bc", "gpu-graph-color", "gpu-pagerank", "gpu-spmv"):
         p = os.path.join(PKG_DIR, exe)
         assert os.path.exists(p) and os.access(p, os.X_OK)
 
