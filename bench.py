@@ -335,7 +335,8 @@ def pr_bench(npa, ctx, args):
                    "iterations": it, "device_ms": round(dev_ms, 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_pr_contrib+k_pr_pull_light+k_pr_pull_heavy (host loop)", "bytes_per_launch": b_alg},
+                     "kernel": "k_pr_contrib+k_pr_pull_light+k_pr_pull_wave+k_pr_pull_huge (host loop)",
+                     "bytes_per_launch": b_alg},
         "cpu_baseline": cpu, "exact": exact,
     }
     print(json.dumps(line), flush=True)
@@ -384,7 +385,7 @@ def spmv_bench(npa, ctx, args):
                    "wall_ms_with_host_copies": round(wall_ms, 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_spmv_heavy_list+k_spmv_light+k_spmv_heavy", "bytes_per_launch": b_alg},
+                     "kernel": "k_pr_bins+k_spmv_light+k_spmv_wave+k_spmv_huge", "bytes_per_launch": b_alg},
         "cpu_baseline": cpu, "exact": exact,
     }
     print(json.dumps(line), flush=True)

@@ -133,6 +133,11 @@ struct np_ctx {
     long long snap_n = 0;
     int relabel = 2;  // NP_CTX_RELABEL: 0 never, 1 always, 2 when in-degree is skewed
     int pack = 1;     // NP_CTX_PACK: batch instances sharing a graph go to source-packed states
+    // grow-only device workspace for PageRank and SpMV (their buffers are carved from it, so
+    // repeated calls allocate nothing once it is large enough) and a small pinned readback
+    char *ws = nullptr;
+    size_t ws_bytes = 0;
+    unsigned long long *h_small = nullptr;  // 4 words
 };
 
 struct np_graph {
@@ -533,6 +538,7 @@ int np_create(int device, np_ctx **out) {
     if (e == hipSuccess) e = hipEventCreate(&c->ev1);
     if (e == hipSuccess) e = dalloc(&c->ctrl, 1);
     if (e == hipSuccess) e = hipHostMalloc((void **)&c->h_ctrl, sizeof(Ctrl), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&c->h_small, 4 * sizeof(unsigned long long), hipHostMallocDefault);
     if (e == hipSuccess) e = dalloc(&c->trace, TRACE_CAP);
     if (e == hipSuccess) e = dalloc(&c->model, NP_MLP_NPARAM);
     if (e == hipSuccess) e = dalloc(&c->d_work2, 4);
@@ -587,6 +593,8 @@ int np_destroy(np_ctx *c) {
     dfree(c->ctrl); dfree(c->trace); dfree(c->model); dfree(c->adam_m); dfree(c->adam_v);
     dfree(c->d_work2); dfree(c->snap_dist); dfree(c->snap_bm);
     if (c->h_ctrl) (void)hipHostFree(c->h_ctrl);
+    if (c->h_small) (void)hipHostFree(c->h_small);
+    if (c->ws) (void)hipFree(c->ws);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
@@ -980,23 +988,83 @@ int np_color(np_ctx *c, np_graph *g, int32_t method, int32_t *color_out, np_stat
     return NP_OK;
 }
 
-// ---- PageRank ----------------------------------------------------------------------------
+// ---- PageRank and SpMV ----------------------------------------------------------------------
+}  // extern "C"
+
+// Buffers carved from the context's workspace: a first pass with base == nullptr sizes the
+// layout, the second hands out pointers (each 256-byte aligned).
+struct Carve {
+    char *base = nullptr;
+    size_t off = 0;
+    template <class T>
+    T *take(size_t count) {
+        off = (off + 255) & ~(size_t)255;
+        T *p = base ? reinterpret_cast<T *>(base + off) : nullptr;
+        off += sizeof(T) * (count ? count : 1);
+        return p;
+    }
+};
+
+// The context's workspace, grown to at least `bytes` (its old contents are dropped).
+static hipError_t ctx_workspace(np_ctx *c, size_t bytes, char **p) {
+    if (bytes > c->ws_bytes) {
+        if (c->ws) (void)hipFree(c->ws);
+        c->ws = nullptr;
+        c->ws_bytes = 0;
+        hipError_t e = hipMalloc((void **)&c->ws, bytes);
+        if (e != hipSuccess) return e;
+        c->ws_bytes = bytes;
+    }
+    *p = c->ws;
+    return hipSuccess;
+}
+
+// Row-bin storage (medium rows, huge rows, their chunks) for n rows holding m entries; the
+// three counts are adjacent, cleared by bins_clear.
+static void bins_carve(Carve &cv, RowBins &rb, int n, long long m) {
+    const size_t nh = (size_t)(m / PR_HUGE) + 1, nc = (size_t)pr_chunk_cap(m);
+    rb.med = cv.take<int>((size_t)n);
+    rb.huge = cv.take<int>(nh);
+    rb.hc0 = cv.take<int>(nh);
+    rb.crow = cv.take<int>(nc);
+    rb.ck = cv.take<int>(nc);
+    rb.part = cv.take<double>(nc);
+    rb.nmed = cv.take<int>(3);
+    rb.nhuge = rb.nmed ? rb.nmed + 1 : nullptr;
+    rb.nchunk = rb.nmed ? rb.nmed + 2 : nullptr;
+}
+static hipError_t bins_clear(const RowBins &rb, hipStream_t s) { return hipMemsetAsync(rb.nmed, 0, 3 * sizeof(int), s); }
+
 // PAGERANK_GPU (apps/pagerank/src/gpu/pagerank_wrapper.cu:199-244, 565-661): damping 0.5, total
 // rank 1000, until no vertex changes by more than 1e-13 (pagerank_wrapper.cu:5-6, pagerank.h:15).
 struct PrBufs {
-    long long *icnt = nullptr, *irow = nullptr;
-    int *cursor = nullptr, *pcol = nullptr, *heavy = nullptr, *nheavy = nullptr;
-    double *rank = nullptr, *nrank = nullptr, *contrib = nullptr, *dsum = nullptr;
-    unsigned long long *dmax = nullptr, *h = nullptr;
-    float *out = nullptr;
-    void *temp = nullptr;
-    ~PrBufs() {
-        dfree(icnt); dfree(irow); dfree(cursor); dfree(pcol); dfree(heavy); dfree(nheavy);
-        dfree(rank); dfree(nrank); dfree(contrib); dfree(dsum); dfree(dmax); dfree(out);
-        if (temp) (void)hipFree(temp);
-        if (h) (void)hipHostFree(h);
-    }
+    long long *irow;
+    int *src, *pcol;
+    unsigned *tkey;
+    RowBins rb;
+    double *rank, *nrank, *contrib, *dsum;
+    unsigned long long *dmax;
+    int *ctl;
+    float *out;
+    void *temp;
 };
+static void pr_carve(Carve &cv, PrBufs &pb, int n, long long m, size_t temp_bytes) {
+    pb.irow = cv.take<long long>((size_t)n + 1);
+    pb.src = cv.take<int>((size_t)m);
+    pb.pcol = cv.take<int>((size_t)m);
+    pb.tkey = cv.take<unsigned>((size_t)m);
+    bins_carve(cv, pb.rb, n, m);
+    pb.rank = cv.take<double>((size_t)n);
+    pb.nrank = cv.take<double>((size_t)n);
+    pb.contrib = cv.take<double>((size_t)n);
+    pb.dsum = cv.take<double>(2);
+    pb.dmax = cv.take<unsigned long long>(2);
+    pb.ctl = cv.take<int>(2);
+    pb.out = cv.take<float>((size_t)n);
+    pb.temp = cv.take<char>(temp_bytes);
+}
+
+extern "C" {
 
 int np_pagerank(np_ctx *c, np_graph *g, int32_t max_iters, float *rank_out, np_stats *stats) {
     if (!c || !g || g->ctx != c || !rank_out || max_iters < 0) { set_error("bad arguments"); return NP_ERR_ARG; }
@@ -1006,52 +1074,58 @@ int np_pagerank(np_ctx *c, np_graph *g, int32_t max_iters, float *rank_out, np_s
     np_stats st{};
     st.drive = NP_DRIVE_HOSTLOOP;
     if (n == 0) { if (stats) *stats = st; return NP_OK; }
-    PrBufs pb;
-    HIPCHK(dalloc(&pb.icnt, (size_t)n + 1));
-    HIPCHK(dalloc(&pb.irow, (size_t)n + 1));
-    HIPCHK(dalloc(&pb.cursor, (size_t)n));
-    HIPCHK(dalloc(&pb.pcol, (size_t)(g->m > 0 ? g->m : 1)));
-    HIPCHK(dalloc(&pb.heavy, (size_t)n));
-    HIPCHK(dalloc(&pb.nheavy, 1));
-    HIPCHK(dalloc(&pb.rank, (size_t)n));
-    HIPCHK(dalloc(&pb.nrank, (size_t)n));
-    HIPCHK(dalloc(&pb.contrib, (size_t)n));
-    HIPCHK(dalloc(&pb.dsum, 1));
-    HIPCHK(dalloc(&pb.dmax, 1));
-    HIPCHK(dalloc(&pb.out, (size_t)n));
-    HIPCHK(hipHostMalloc((void **)&pb.h, sizeof(unsigned long long)));
+    const long long m = g->m;
+    int end_bit = 1;  // key bits covering vertex ids 0..n-1
+    while (end_bit < 31 && (1ll << end_bit) < n) ++end_bit;
+    const unsigned *tkey_in = reinterpret_cast<const unsigned *>(g->col);
     size_t temp_bytes = 0;
-    HIPCHK(pr_scan(nullptr, &temp_bytes, pb.icnt, pb.irow, n, c->stream));
-    HIPCHK(hipMalloc(&pb.temp, temp_bytes ? temp_bytes : 1));
+    HIPCHK(pr_sort(nullptr, &temp_bytes, tkey_in, nullptr, nullptr, nullptr, m, end_bit, c->stream));
+    PrBufs pb{};
+    Carve cv;
+    pr_carve(cv, pb, n, m, temp_bytes);
+    HIPCHK(ctx_workspace(c, cv.off, &cv.base));
+    cv.off = 0;
+    pr_carve(cv, pb, n, m, temp_bytes);
     HIPCHK(hipEventRecord(c->ev0, c->stream));
-    HIPCHK(hipMemsetAsync(pb.icnt, 0, sizeof(long long) * ((size_t)n + 1), c->stream));
-    HIPCHK(hipMemsetAsync(pb.cursor, 0, sizeof(int) * (size_t)n, c->stream));
-    HIPCHK(hipMemsetAsync(pb.nheavy, 0, sizeof(int), c->stream));
-    HIPCHK(launch_pr_count(g->row, g->col, n, pb.icnt, c->stream));
-    size_t tb = temp_bytes;
-    HIPCHK(pr_scan(pb.temp, &tb, pb.icnt, pb.irow, n, c->stream));
-    HIPCHK(launch_pr_fill(g->row, g->col, n, pb.irow, pb.cursor, pb.pcol, c->stream));
-    HIPCHK(launch_pr_heavy(pb.irow, n, pb.heavy, pb.nheavy, c->stream));
+    HIPCHK(bins_clear(pb.rb, c->stream));
+    if (m > 0) {
+        HIPCHK(launch_pr_sources(g->row, n, pb.src, c->stream));
+        size_t tb = temp_bytes;
+        HIPCHK(pr_sort(pb.temp, &tb, tkey_in, pb.tkey, pb.src, pb.pcol, m, end_bit, c->stream));
+    }
+    HIPCHK(launch_pr_offsets(pb.tkey, m, n, pb.irow, c->stream));
+    HIPCHK(launch_pr_bins(pb.irow, n, m, pb.rb, c->stream));
     HIPCHK(launch_pr_init(pb.rank, n, total / n, c->stream));
+    HIPCHK(hipMemsetAsync(pb.ctl, 0, 2 * sizeof(int), c->stream));
     const double rw = (1.0 - damping) * total / n;
     const int cap = max_iters > 0 ? max_iters : 100000;
-    int it = 0;
+    // Iterations go out PR_BATCH at a time; the device ends the run (see launch_pr_iteration) and
+    // the host reads the iteration count once per batch instead of the change once per iteration.
+    constexpr int PR_BATCH = 8;
+    double *r[2] = {pb.rank, pb.nrank};
+    int launched = 0, it = 0;
     for (;;) {
-        if (it >= cap) { set_error("PageRank did not converge in %d iterations", cap); return NP_ERR_NOT_CONVERGED; }
-        HIPCHK(hipMemsetAsync(pb.dsum, 0, sizeof(double), c->stream));
-        HIPCHK(hipMemsetAsync(pb.dmax, 0, sizeof(unsigned long long), c->stream));
-        HIPCHK(launch_pr_iteration(g->row, pb.irow, pb.pcol, pb.heavy, pb.nheavy, pb.rank, pb.nrank, pb.contrib, n,
-                                   damping, rw, pb.dsum, pb.dmax, c->num_cus, c->stream));
-        HIPCHK(hipMemcpyAsync(pb.h, pb.dmax, sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
+        if (launched >= cap) { set_error("PageRank did not converge in %d iterations", cap); return NP_ERR_NOT_CONVERGED; }
+        const int k = std::min(PR_BATCH, cap - launched);
+        for (int j = 0; j < k; ++j, ++launched)
+            HIPCHK(launch_pr_iteration(launched, pb.ctl, g->row, pb.irow, pb.pcol, pb.rb, r[launched & 1],
+                                       r[(launched + 1) & 1], pb.contrib, n, damping, rw, eps, pb.dsum, pb.dmax,
+                                       c->num_cus, c->stream));
+        unsigned long long *h = c->h_small;
+        HIPCHK(hipMemcpyAsync(h, pb.ctl, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(h + 1, pb.dmax, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
         int rc = ctx_sync(c);
         if (rc) return rc;
-        std::swap(pb.rank, pb.nrank);
-        ++it;
-        double dmax;
-        std::memcpy(&dmax, pb.h, sizeof dmax);
-        if (!(dmax > eps)) break;  // no vertex changed by more than EPSILON
+        int hc[2];
+        std::memcpy(hc, h, sizeof hc);
+        if (hc[0]) { it = hc[1]; break; }  // iteration hc[1] changed no vertex by more than EPSILON
+        if (launched >= cap) {             // the last allowed iteration ran: its own change decides
+            double dlast;
+            std::memcpy(&dlast, h + 1 + ((launched - 1) & 1), sizeof dlast);
+            if (!(dlast > eps)) { it = launched; break; }
+        }
     }
-    HIPCHK(launch_pr_to_float(pb.rank, n, pb.out, c->stream));
+    HIPCHK(launch_pr_to_float(r[it & 1], n, pb.out, c->stream));
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     HIPCHK(hipMemcpyAsync(rank_out, pb.out, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, c->stream));
     int rc = ctx_sync(c);
@@ -1075,22 +1149,24 @@ int np_spmv(np_ctx *c, np_graph *g, const double *data, const double *x, double 
     st.drive = NP_DRIVE_HOSTLOOP;
     if (n == 0) { if (stats) *stats = st; return NP_OK; }
     double *d_data = nullptr, *d_x = nullptr, *d_y = nullptr;
-    int *heavy = nullptr, *nheavy = nullptr;
-    struct Free {
-        double **a, **b, **c; int **h, **nh;
-        ~Free() { dfree(*a); dfree(*b); dfree(*c); dfree(*h); dfree(*nh); }
-    } fr{&d_data, &d_x, &d_y, &heavy, &nheavy};
-    HIPCHK(dalloc(&d_data, (size_t)(m > 0 ? m : 1)));
-    HIPCHK(dalloc(&d_x, (size_t)n));
-    HIPCHK(dalloc(&d_y, (size_t)n));
-    HIPCHK(dalloc(&heavy, (size_t)n));
-    HIPCHK(dalloc(&nheavy, 1));
+    RowBins rb{};
+    Carve cv;
+    for (int pass = 0; pass < 2; ++pass) {  // size, then carve from the context's workspace
+        if (pass) {
+            HIPCHK(ctx_workspace(c, cv.off, &cv.base));
+            cv.off = 0;
+        }
+        d_data = cv.take<double>((size_t)m);
+        d_x = cv.take<double>((size_t)n);
+        d_y = cv.take<double>((size_t)n);
+        bins_carve(cv, rb, n, m);
+    }
     if (m > 0) HIPCHK(hipMemcpyAsync(d_data, data, sizeof(double) * (size_t)m, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(d_x, x, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemsetAsync(nheavy, 0, sizeof(int), c->stream));
+    HIPCHK(bins_clear(rb, c->stream));
     HIPCHK(hipEventRecord(c->ev0, c->stream));
-    HIPCHK(launch_spmv_heavy_list(g->row, n, heavy, nheavy, c->stream));
-    HIPCHK(launch_spmv(g->row, g->col, d_data, d_x, d_y, n, heavy, nheavy, c->num_cus, c->stream));
+    HIPCHK(launch_spmv_bins(g->row, n, m, rb, c->stream));
+    HIPCHK(launch_spmv(g->row, g->col, d_data, d_x, d_y, n, rb, c->num_cus, c->stream));
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     HIPCHK(hipMemcpyAsync(y, d_y, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, c->stream));
     int rc = ctx_sync(c);

@@ -162,22 +162,39 @@ hipError_t launch_color_round(const int *row, const int *col, int n, int *color,
 hipError_t color_scan(void *temp, size_t *temp_bytes, const long long *in, long long *out, int n, hipStream_t s);
 
 // ---- pagerank.hip: PageRank over the in-edge CSR ------------------------------------------
-hipError_t launch_pr_count(const int *row, const int *col, int n, long long *icnt, hipStream_t s);
-hipError_t launch_pr_fill(const int *row, const int *col, int n, const long long *irow, int *cursor, int *pcol,
-                          hipStream_t s);
-hipError_t launch_pr_heavy(const long long *irow, int n, int *heavy, int *nheavy, hipStream_t s);
+// Rows by length: < PR_HEAVY entries PR_GROUP lanes each; < PR_HUGE one wavefront each (med);
+// longer rows (huge) in PR_CHUNK-entry chunks, one wavefront per chunk, partial sums in part[].
+constexpr int PR_GROUP = 8;
+constexpr int PR_HEAVY = 32;
+constexpr int PR_CHUNK = 2048;
+constexpr int PR_HUGE = 2 * PR_CHUNK;
+struct RowBins {
+    int *med, *nmed;              // medium rows [n]; count
+    int *huge, *hc0, *nhuge;      // huge rows and their first chunk [m / PR_HUGE + 1]; count
+    int *crow, *ck, *nchunk;      // chunk -> row, chunk index [pr_chunk_cap(m)]; count
+    double *part;                 // per-chunk partial sums [pr_chunk_cap(m)]
+};
+long long pr_chunk_cap(long long m);
+// In-edge CSR: src[e] = source of edge e; sort (col, src) pairs; offsets from the sorted targets.
+hipError_t launch_pr_sources(const int *row, int n, int *src, hipStream_t s);
+hipError_t pr_sort(void *temp, size_t *temp_bytes, const unsigned *key_in, unsigned *key_out, const int *val_in,
+                   int *val_out, long long m, int end_bit, hipStream_t s);
+hipError_t launch_pr_offsets(const unsigned *key, long long m, int n, long long *irow, hipStream_t s);
 hipError_t launch_pr_init(double *rank, int n, double r0, hipStream_t s);
-// One iteration rank -> nrank (contrib: scratch; *dsum and *dmax must be zero on entry).
-hipError_t launch_pr_iteration(const int *row, const long long *irow, const int *pcol, const int *heavy,
-                               const int *nheavy, const double *rank, double *nrank, double *contrib, int n,
-                               double damping, double rw, double *dsum, unsigned long long *dmax, int num_cus,
-                               hipStream_t s);
+// Bins over irow (counts *nmed, *nhuge, *nchunk zero on entry).
+hipError_t launch_pr_bins(const long long *irow, int n, long long m, const RowBins &rb, hipStream_t s);
+// Iteration i, rank -> nrank, decided on the device: ctl[0] (done) is set when iteration i - 1
+// changed no vertex by more than eps, and then nothing runs; else ctl[1] = i + 1.  Sums of
+// iteration i in dsum2[i & 1], dmax2[i & 1] (cleared by the iteration itself); contrib: scratch.
+hipError_t launch_pr_iteration(int i, int *ctl, const int *row, const long long *irow, const int *pcol,
+                               const RowBins &rb, const double *rank, double *nrank, double *contrib, int n,
+                               double damping, double rw, double eps, double *dsum2, unsigned long long *dmax2,
+                               int num_cus, hipStream_t s);
 hipError_t launch_pr_to_float(const double *rank, int n, float *out, hipStream_t s);
-// SpMV y = A x over the graph's CSR (rows of >= 32 entries listed in heavy[*nheavy]).
-hipError_t launch_spmv_heavy_list(const int *row, int n, int *heavy, int *nheavy, hipStream_t s);
+// SpMV y = A x over the graph's CSR, with row bins built by launch_spmv_bins.
+hipError_t launch_spmv_bins(const int *row, int n, long long m, const RowBins &rb, hipStream_t s);
 hipError_t launch_spmv(const int *row, const int *col, const double *data, const double *x, double *y, int n,
-                       const int *heavy, const int *nheavy, int num_cus, hipStream_t s);
-hipError_t pr_scan(void *temp, size_t *temp_bytes, const long long *in, long long *out, int n, hipStream_t s);
+                       const RowBins &rb, int num_cus, hipStream_t s);
 
 // ---- gen.hip: generators and CSR construction ---------------------------------------
 struct GenOut { int *row; int *col; int *w; };

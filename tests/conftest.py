@@ -61,5 +61,27 @@ def random_csr(rng, n, m, wmin=1, wmax=100, self_loops=True):
     return row.astype(np.int32), dst.astype(np.int32), w.astype(np.int32)
 
 
+def hub_csr(rng, n, m, hub_sizes, inward=False):
+    """Random multigraph plus rows (inward=False) or in-edge lists (inward=True) of exactly
+    hub_sizes[i] entries at vertex i, on top of m random edges (hub vertices get no random
+    entries of that kind): rows longer than one wavefront's share, split into chunks."""
+    src = rng.integers(len(hub_sizes), n, m)
+    dst = rng.integers(0, n, m)
+    if inward:
+        dst = rng.integers(len(hub_sizes), n, m)
+        src = rng.integers(0, n, m)
+    hs, hd = [src], [dst]
+    for i, k in enumerate(hub_sizes):
+        other = rng.integers(0, n, k)
+        hs.append(other if inward else np.full(k, i))
+        hd.append(np.full(k, i) if inward else other)
+    src, dst = np.concatenate(hs), np.concatenate(hd)
+    order = np.argsort(src, kind="stable")
+    src, dst = src[order], dst[order]
+    row = np.zeros(n + 1, np.int64)
+    np.add.at(row, src + 1, 1)
+    return np.cumsum(row).astype(np.int32), dst.astype(np.int32)
+
+
 def load_golden(name):
     return dict(np.load(os.path.join(GOLDEN, name), allow_pickle=False))

@@ -1,20 +1,26 @@
 """np_pagerank (PAGERANK_GPU) against the oracle: float64 iterations on both sides, the GPU's
-ranks returned as float32; bar |gpu - oracle| <= 1e-5 * |oracle| + 1e-9 and the same
-iteration count within one (summation order can move the last change across 1e-13)."""
+ranks returned as float32; bar |gpu - oracle| <= 1e-5 * |oracle| + 1e-9.  Both runs stop past
+convergence: the oracle's ranks after min(gpu, oracle) iterations equal its final ranks to 1e-12
+relative.  The counts agree within one where rounding cannot reach the 1e-13 stop (16 ulp of
+the largest rank below it); with hub ranks ~140 (1e-13 is 3.5 ulp) the last changes are
+summation-order noise and the counts may differ by a few."""
 import os
 import subprocess
 
 import numpy as np
 import pytest
 
-from conftest import GOLDEN, PKG_DIR, load_golden, random_csr
+from conftest import GOLDEN, PKG_DIR, hub_csr, load_golden, random_csr
 
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("kind", ["rmat", "er", "grid", "multigraph"])
+@pytest.mark.parametrize("kind", ["rmat", "er", "grid", "multigraph", "hubs"])
 def test_pagerank_matches_oracle(npa, ctx, orc, kind):
-    if kind == "rmat":
+    if kind == "hubs":                                           # in-degrees 4095..20000: chunked pulls
+        row, col = hub_csr(np.random.default_rng(76), 6000, 30000, [4095, 4096, 4097, 9000, 20000], inward=True)
+        G = npa.Graph.from_csr(ctx, row, col, np.ones(len(col), np.int32))
+    elif kind == "rmat":
         G = npa.Graph.rmat(ctx, 13, 16, wmin=1, wmax=1, seed=71)
     elif kind == "er":
         G = npa.Graph.er(ctx, 5000, 40000, 1, 1, seed=72)
@@ -29,7 +35,10 @@ def test_pagerank_matches_oracle(npa, ctx, orc, kind):
     ref, it = orc.pagerank(row, col)
     got, st = ctx.pagerank(G)
     assert np.allclose(got, ref, rtol=1e-5, atol=1e-9), float(np.max(np.abs(got - ref) / ref))
-    assert abs(st.iterations - it) <= 1, (st.iterations, it)
+    early, _ = orc.pagerank(row, col, max_iters=min(st.iterations, it))
+    assert np.allclose(early, ref, rtol=1e-12, atol=0), (st.iterations, it)
+    if float(ref.max()) * 2.0 ** -52 * 16 < 1e-13:
+        assert abs(st.iterations - it) <= 1, (st.iterations, it)
     assert abs(float(got.astype(np.float64).sum()) - 1000.0) < 1e-2
     G.free()
 

@@ -6,14 +6,21 @@ import subprocess
 import numpy as np
 import pytest
 
-from conftest import GOLDEN, PKG_DIR, load_golden, random_csr
+from conftest import GOLDEN, PKG_DIR, hub_csr, load_golden, random_csr
 
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("kind", ["rmat", "grid", "multigraph"])
+# rows of 4095 / 4096 / 4097 / 9000 / 20000 entries: the last medium row and chunked rows
+HUBS = [4095, 4096, 4097, 9000, 20000]
+
+
+@pytest.mark.parametrize("kind", ["rmat", "grid", "multigraph", "hubs"])
 def test_spmv_matches_oracle(npa, ctx, orc, kind):
-    if kind == "rmat":
+    if kind == "hubs":
+        row, col = hub_csr(np.random.default_rng(85), 6000, 30000, HUBS)
+        G = npa.Graph.from_csr(ctx, row, col, np.ones(len(col), np.int32))
+    elif kind == "rmat":
         G = npa.Graph.rmat(ctx, 14, 16, wmin=1, wmax=1, seed=81)
     elif kind == "grid":
         G = npa.Graph.grid(ctx, 64, 80, None, 1, 1, seed=82)
