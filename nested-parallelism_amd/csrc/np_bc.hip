@@ -131,10 +131,20 @@ __global__ __launch_bounds__(256) void k_bc_expand(BcDev b, const int *ent_v, co
             if (live) {
                 const int w = b.col[e0 + (int)(r >> lp)];
                 const long long sw = ((long long)w << b.ks) + src;
-                const int old = atomicMin(&b.lvl[sw], d + 1);
-                if (old >= d + 1) {  // w is at level d + 1 for this source: v is a predecessor
+                // Levels up to d were set by earlier launches and are final, so a plain read
+                // settles most edges without an atomic: <= d, w was reached before (nothing to
+                // do); d + 1, w is on the next level already (only the path count).  Only an
+                // unseen w needs the atomicMin; a read racing with this launch's writes sees
+                // unseen or d + 1, never a wrong level.
+                const int cur = b.lvl[sw];
+                if (cur == d + 1) {
                     atomicAdd(&b.sig[sw], b.sig[sv]);
-                    if (old > d + 1) atomicOr(&b.nmask[w], 1u << src);  // first reached
+                } else if (cur > d + 1) {
+                    const int old = atomicMin(&b.lvl[sw], d + 1);
+                    if (old >= d + 1) {  // w is at level d + 1 for this source: v is a predecessor
+                        atomicAdd(&b.sig[sw], b.sig[sv]);
+                        if (old > d + 1) atomicOr(&b.nmask[w], 1u << src);  // first reached
+                    }
                 }
             }
         } else {
