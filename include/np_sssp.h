@@ -304,8 +304,9 @@ int np_read_graph_stream(int32_t fmt, void *fp, int32_t *n, int64_t *m, int32_t 
  * BC_GPU's loop does), unnormalised, on the directed graph with unit edge lengths; parallel
  * edges count as distinct shortest paths, as the reference's per-edge path-count update does.
  * bc_out[n] receives float values (the reference's float *bc), accumulated in double.  Sources
- * run in batches of up to 32 on the same frontier machinery as the source-packed SSSP, host-loop
- * driven (two 4-byte readbacks per level).  stats: iterations = levels summed over batches,
+ * run in batches of up to 32 on the same frontier machinery as the source-packed SSSP; forward
+ * levels go out in runs as long as the previous batch's depth + 1, with one readback of the
+ * runs' (first entry, count) pairs per run.  stats: iterations = levels summed over batches,
  * frontier_total = (vertex, source mask) entries, kernel_ms = device time of the whole call. */
 int np_bc(np_ctx *ctx, np_graph *g, const int32_t *srcs, int32_t nsrc, float *bc_out, np_stats *stats);
 

@@ -755,17 +755,19 @@ int np_reached_work(np_ctx *c, np_graph *g, const int32_t *d_dist, int32_t inf, 
 // up to 32; per batch a level-synchronous forward BFS with path counts records every level's
 // entries, then the backward sweep replays them from the deepest level up.
 struct BcBufs {
-    int *lvl = nullptr, *ent_v = nullptr, *owner = nullptr, *count = nullptr, *srcs = nullptr, *err = nullptr;
+    int *lvl = nullptr, *ent_v = nullptr, *owner = nullptr, *srcs = nullptr, *err = nullptr;
+    int *ectl = nullptr, *lv = nullptr, *h_lv = nullptr;
     unsigned *ent_m = nullptr, *nmask = nullptr;
     double *sig = nullptr, *del = nullptr, *acc = nullptr;
     long long *ent_c = nullptr, *cpref = nullptr, *h_total = nullptr;
     float *out = nullptr;
     void *temp = nullptr;
     ~BcBufs() {
-        dfree(lvl); dfree(ent_v); dfree(owner); dfree(count); dfree(srcs); dfree(err); dfree(ent_m); dfree(nmask);
-        dfree(sig); dfree(del); dfree(acc); dfree(ent_c); dfree(cpref); dfree(out);
+        dfree(lvl); dfree(ent_v); dfree(owner); dfree(srcs); dfree(err); dfree(ent_m); dfree(nmask);
+        dfree(sig); dfree(del); dfree(acc); dfree(ent_c); dfree(cpref); dfree(out); dfree(ectl); dfree(lv);
         if (temp) (void)hipFree(temp);
         if (h_total) (void)hipHostFree(h_total);
+        if (h_lv) (void)hipHostFree(h_lv);
     }
 };
 
@@ -805,13 +807,21 @@ int np_bc(np_ctx *c, np_graph *g, const int32_t *srcs, int32_t nsrc, float *bc_o
     HIPCHK(dalloc(&bb.ent_c, (size_t)n + 1));
     HIPCHK(dalloc(&bb.cpref, (size_t)n + 1));
     HIPCHK(dalloc(&bb.owner, (size_t)cap_owner));
-    HIPCHK(dalloc(&bb.count, 1));
     HIPCHK(dalloc(&bb.err, 1));
     HIPCHK(hipMemsetAsync(bb.err, 0, sizeof(int), c->stream));
     HIPCHK(dalloc(&bb.srcs, 32));
     HIPCHK(dalloc(&bb.acc, (size_t)n));
     HIPCHK(dalloc(&bb.out, (size_t)n));
     HIPCHK(hipHostMalloc((void **)&bb.h_total, 2 * sizeof(long long)));
+    // Forward levels go out several at a time with the entry counts kept on the device; the host
+    // reads each run's (first entry, count) pairs once.  A run is as long as the previous source
+    // batch's depth + 1 (depths barely vary on one graph), at most BC_RUN, then 2 at a time;
+    // levels past the last are empty and cost a few small launches.
+    constexpr int BC_RUN = 64;
+    HIPCHK(dalloc(&bb.ectl, 2));  // first entry of the current level, its entry count
+    HIPCHK(dalloc(&bb.lv, 2 * ((size_t)n + BC_RUN + 1)));
+    HIPCHK(hipHostMalloc((void **)&bb.h_lv, 2 * BC_RUN * sizeof(int)));
+    int run = 8;
     size_t temp_bytes = 0;
     HIPCHK(bc_scan(nullptr, &temp_bytes, bb.ent_c, bb.cpref, n, c->stream));
     HIPCHK(hipMalloc(&bb.temp, temp_bytes ? temp_bytes : 1));
@@ -819,12 +829,12 @@ int np_bc(np_ctx *c, np_graph *g, const int32_t *srcs, int32_t nsrc, float *bc_o
     BcDev b{g->row, g->col, n, ks, bb.lvl, bb.sig, bb.del, bb.nmask};
     // chunks of the ne entries at ev/em (ent_c holds their counts): prefix and owners, all on the
     // stream (the expansion reads the chunk total from cpref[ne])
-    auto chunks = [&](int ne, const int *ev, const unsigned *em, bool counted) -> int {
-        if (!counted) HIPCHK(launch_bc_chunks(b, ev, em, ne, bb.ent_c, c->stream));
+    auto chunks = [&](int ne, const int *ev, const unsigned *em) -> int {
+        HIPCHK(launch_bc_chunks(b, ev, em, ne, bb.ent_c, c->stream));
         HIPCHK(hipMemsetAsync(bb.ent_c + ne, 0, sizeof(long long), c->stream));
         size_t tb = temp_bytes;
         HIPCHK(bc_scan(bb.temp, &tb, bb.ent_c, bb.cpref, ne, c->stream));
-        HIPCHK(launch_bc_owner(bb.cpref, ne, bb.owner, cap_owner, bb.err, c->stream));
+        HIPCHK(launch_bc_owner(bb.cpref, ne, nullptr, bb.owner, cap_owner, bb.err, c->stream));
         return NP_OK;
     };
     HIPCHK(hipEventRecord(c->ev0, c->stream));
@@ -836,30 +846,46 @@ int np_bc(np_ctx *c, np_graph *g, const int32_t *srcs, int32_t nsrc, float *bc_o
         HIPCHK(launch_bc_sources(b, bb.srcs, np_, c->stream));
         levels.clear();
         long long E = 0;
-        for (int d = 0;; ++d) {
-            HIPCHK(hipMemsetAsync(bb.count, 0, sizeof(int), c->stream));
-            HIPCHK(launch_bc_compact(b, bb.ent_v + E, bb.ent_m + E, bb.ent_c, bb.count, c->stream));
-            HIPCHK(hipMemcpyAsync(bb.h_total + 1, bb.count, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemsetAsync(bb.ectl, 0, 2 * sizeof(int), c->stream));
+        for (int d0 = 0, last = -1, k = run; last < 0; d0 += k, k = 2) {
+            if (d0 > n) { set_error("more than n BFS levels"); return NP_ERR_NOT_CONVERGED; }
+            for (int d = d0; d < d0 + k; ++d) {
+                // chunk counts of the level's entries, zero up to n: the scan of n + 1 counts needs
+                // no host-side entry count (the compaction reads all n vertices anyway)
+                HIPCHK(hipMemsetAsync(bb.ent_c, 0, sizeof(long long) * ((size_t)n + 1), c->stream));
+                HIPCHK(launch_bc_compact(b, bb.ent_v, bb.ent_m, bb.ectl, bb.ent_c, bb.ectl + 1, c->stream));
+                size_t tb = temp_bytes;
+                HIPCHK(bc_scan(bb.temp, &tb, bb.ent_c, bb.cpref, n, c->stream));
+                HIPCHK(launch_bc_owner(bb.cpref, n, bb.ectl + 1, bb.owner, cap_owner, bb.err, c->stream));
+                HIPCHK(launch_bc_expand(b, true, bb.ent_v, bb.ent_m, bb.ectl, bb.cpref, bb.owner, n, cap_owner, d,
+                                        c->num_cus, c->stream));
+                HIPCHK(launch_bc_advance(bb.ectl, bb.ectl + 1, bb.lv, d, c->stream));
+            }
+            HIPCHK(hipMemcpyAsync(bb.h_lv, bb.lv + 2 * (size_t)d0, 2 * (size_t)k * sizeof(int), hipMemcpyDeviceToHost,
+                                  c->stream));
             int rc = ctx_sync(c);
             if (rc) return rc;
-            int ne = 0;
-            std::memcpy(&ne, bb.h_total + 1, sizeof(int));
-            if (ne <= 0) break;
-            if (E + ne > cap_e) { set_error("entry capacity exceeded"); return NP_ERR_CAPACITY; }
-            levels.push_back({E, ne});
-            if ((rc = chunks(ne, bb.ent_v + E, bb.ent_m + E, true))) return rc;
-            HIPCHK(launch_bc_expand(b, true, bb.ent_v + E, bb.ent_m + E, bb.cpref, bb.owner, ne, cap_owner, d,
-                                    c->num_cus, c->stream));
-            E += ne;
+            for (int j = 0; j < k; ++j) {
+                const int ne = bb.h_lv[2 * j + 1];
+                if (ne <= 0) { last = d0 + j; break; }
+                levels.push_back({(long long)bb.h_lv[2 * j], ne});
+                E += ne;
+            }
         }
-        // the deepest level has no successors (dependency 0); level 0 holds the sources
+        run = std::min<int>(BC_RUN, (int)levels.size() + 1);
+        // the deepest level has no successors (dependency 0): its slots only take their factor
+        // 1 / sigma (k_bc_accum adds zeros to bc); level 0 holds the sources
+        if (levels.size() >= 2) {
+            const long long E0 = levels.back().first;
+            HIPCHK(launch_bc_accum(b, bb.ent_v + E0, bb.ent_m + E0, levels.back().second, bb.acc, c->stream));
+        }
         for (int li = (int)levels.size() - 2; li >= 1; --li) {
             const long long E0 = levels[(size_t)li].first;
             const int ne = levels[(size_t)li].second;
-            int rc = chunks(ne, bb.ent_v + E0, bb.ent_m + E0, false);
+            int rc = chunks(ne, bb.ent_v + E0, bb.ent_m + E0);
             if (rc) return rc;
-            HIPCHK(launch_bc_expand(b, false, bb.ent_v + E0, bb.ent_m + E0, bb.cpref, bb.owner, ne, cap_owner, li,
-                                    c->num_cus, c->stream));
+            HIPCHK(launch_bc_expand(b, false, bb.ent_v + E0, bb.ent_m + E0, nullptr, bb.cpref, bb.owner, ne, cap_owner,
+                                    li, c->num_cus, c->stream));
             HIPCHK(launch_bc_accum(b, bb.ent_v + E0, bb.ent_m + E0, ne, bb.acc, c->stream));
         }
         st.iterations += (int)levels.size();

@@ -5,9 +5,10 @@
 // kernels of bc_fp_kernel.cu:429-489 (forward: a vertex reached at level d+1 adds the path count
 // of every level-d predecessor, sigma[w] += sigma[u]) and bc_bp_kernel.cu:4-36 (backward:
 // delta[v] += sigma[v] / sigma[w] * (1 + delta[w]) over the edges v -> w into the next level;
-// bc[v] += delta[v] for v != source).  The reference keeps an n x n predecessor byte matrix;
-// here an edge u -> w is a predecessor edge exactly when level[w] = level[u] + 1, which is what
-// that matrix records, so no n^2 state exists.
+// bc[v] += delta[v] for v != source).  Here the factor (1 + delta[w]) / sigma[w] is formed once
+// per slot when w's level is final and read as one value by the level above.  The reference
+// keeps an n x n predecessor byte matrix; here an edge u -> w is a predecessor edge exactly
+// when level[w] = level[u] + 1, which is what that matrix records, so no n^2 state exists.
 //
 // Layout: the sources of a batch share rows of 2^ks slots, slot (v << ks) + j for source j, as
 // the source-packed SSSP does: level (int), sigma and delta (double: path counts stay exact
@@ -57,8 +58,11 @@ __global__ void k_bc_sources(BcDev b, const int *srcs, int nsrc) {
     atomicOr(&b.nmask[s], 1u << j);
 }
 
-// Flagged vertices -> entries (v, mask) at ent_v/ent_m + *count, and their chunk counts.
-__global__ void k_bc_compact(BcDev b, int *ent_v, unsigned *ent_m, long long *ent_c, int *count) {
+// Flagged vertices -> entries (v, mask) at ent_v/ent_m + *eoff + (position counted in *count),
+// and their chunk counts at ent_c[position].  Entries of all levels fit in n x sources: every
+// (vertex, source) pair is in one level's entry and every entry holds at least one.
+__global__ void k_bc_compact(BcDev b, int *ent_v, unsigned *ent_m, const int *eoff, long long *ent_c, int *count) {
+    const int E = *eoff;
     for (long long base = (long long)blockIdx.x * blockDim.x; base < b.n; base += (long long)gridDim.x * blockDim.x) {
         const long long v = base + threadIdx.x;
         const unsigned m = v < b.n ? b.nmask[v] : 0u;
@@ -75,8 +79,8 @@ __global__ void k_bc_compact(BcDev b, int *ent_v, unsigned *ent_m, long long *en
             const int p = __popc(m);
             const int lp = p <= 1 ? 0 : 32 - __clz(p - 1);
             const long long items = (long long)(b.row[v + 1] - b.row[v]) << lp;
-            ent_v[pos] = (int)v;
-            ent_m[pos] = m;
+            ent_v[E + pos] = (int)v;
+            ent_m[E + pos] = m;
             ent_c[pos] = (items + 63) >> 6;
         }
     }
@@ -92,10 +96,11 @@ __global__ void k_bc_chunks(BcDev b, const int *ent_v, const unsigned *ent_m, in
     }
 }
 
-// owner[c] = o for every chunk c of entry o (cpref: exclusive prefix of chunk counts, ne + 1),
-// one wavefront per entry.  A level has at most m/2 + ne chunks (deg x 32 / 64 + 1 per entry);
-// a total above cap sets *err and writes nothing past it.
-__global__ void k_bc_owner(const long long *cpref, int ne, int *owner, long long cap, int *err) {
+// owner[c] = o for every chunk c of entry o (cpref: exclusive prefix of chunk counts, ne + 1;
+// ne from *ne_dev when given), one wavefront per entry.  A level has at most m/2 + ne chunks
+// (deg x 32 / 64 + 1 per entry); a total above cap sets *err and writes nothing past it.
+__global__ void k_bc_owner(const long long *cpref, int ne, const int *ne_dev, int *owner, long long cap, int *err) {
+    if (ne_dev) ne = *ne_dev;
     const int lane = bc_lane();
     const long long nw = (long long)gridDim.x * (blockDim.x >> 6);
     if (blockIdx.x == 0 && threadIdx.x == 0 && cpref[ne] > cap) *err = 1;
@@ -106,11 +111,16 @@ __global__ void k_bc_owner(const long long *cpref, int ne, int *owner, long long
 }
 
 // One 64-item chunk per wavefront.  FWD: level d -> d + 1 with path counts; else the
-// dependencies of level d from level d + 1.
+// dependencies of level d from level d + 1.  The level's entries start at ent_v + *eoff (eoff
+// null: at ent_v); cpref[ne] is the chunk total.
 template <bool FWD>
 __global__ __launch_bounds__(256) void k_bc_expand(BcDev b, const int *ent_v, const unsigned *ent_m,
-                                                   const long long *cpref, const int *owner, int ne,
-                                                   long long cap, int d) {
+                                                   const int *eoff, const long long *cpref, const int *owner,
+                                                   int ne, long long cap, int d) {
+    if (eoff) {
+        ent_v += *eoff;
+        ent_m += *eoff;
+    }
     const int lane = bc_lane();
     const long long nchunks = cpref[ne] < cap ? cpref[ne] : cap;
     const long long nw = (long long)gridDim.x * (blockDim.x >> 6);
@@ -152,7 +162,8 @@ __global__ __launch_bounds__(256) void k_bc_expand(BcDev b, const int *ent_v, co
             if (live) {
                 const int w = b.col[e0 + (int)(r >> lp)];
                 const long long sw = ((long long)w << b.ks) + src;
-                if (b.lvl[sw] == d + 1) x = b.sig[sv] / b.sig[sw] * (1.0 + b.del[sw]);
+                // level d + 1 is final: its slots hold (1 + delta) / sigma (k_bc_accum)
+                if (b.lvl[sw] == d + 1) x = b.sig[sv] * b.sig[sw];
             }
             for (int off = 32; off >= (1 << lp) && off > 0; off >>= 1) x += __shfl_xor(x, off);
             if (lane < p && lane == j && x != 0.0) atomicAdd(&b.del[((long long)v << b.ks) + bc_nth_bit(m, lane)], x);
@@ -160,14 +171,34 @@ __global__ __launch_bounds__(256) void k_bc_expand(BcDev b, const int *ent_v, co
     }
 }
 
-// bc[v] += delta of v for every source of its entry at this level (level >= 1: v != source).
+// Once a level's dependencies are final: bc[v] += delta of v for every source of its entry
+// (level >= 1: v != source), and the slot's sigma is replaced by (1 + delta) / sigma, the
+// factor the level above needs from it (one gather there instead of two).  Each (vertex,
+// source) slot is in exactly one level's entries, so it is transformed once, after its own
+// level's sweep; the deepest level (delta 0) is passed through here too.
 __global__ void k_bc_accum(BcDev b, const int *ent_v, const unsigned *ent_m, int ne, double *bc) {
     for (long long o = (long long)blockIdx.x * blockDim.x + threadIdx.x; o < ne; o += (long long)gridDim.x * blockDim.x) {
         const int v = ent_v[o];
         double s = 0.0;
-        for (unsigned m = ent_m[o]; m; m &= m - 1) s += b.del[((long long)v << b.ks) + (__ffs(m) - 1)];
+        for (unsigned m = ent_m[o]; m; m &= m - 1) {
+            const long long k = ((long long)v << b.ks) + (__ffs(m) - 1);
+            const double dl = b.del[k];
+            s += dl;
+            b.sig[k] = (1.0 + dl) / b.sig[k];
+        }
         bc[v] += s;  // one entry per vertex per level
     }
+}
+
+// After level d's forward expansion (one thread): lv[2d], lv[2d + 1] = the level's first entry
+// and entry count; the next level's entries start after them.
+__global__ void k_bc_advance(int *eoff, int *count, int *lv, int d) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const int E = *eoff, ne = *count;
+    lv[2 * d] = E;
+    lv[2 * d + 1] = ne;
+    *eoff = E + ne;
+    *count = 0;
 }
 
 __global__ void k_bc_to_float(const double *bc, int n, float *out) {
@@ -188,9 +219,13 @@ hipError_t launch_bc_sources(const BcDev &b, const int *srcs, int nsrc, hipStrea
     hipLaunchKernelGGL(k_bc_sources, dim3(1), dim3(64), 0, s, b, srcs, nsrc);
     return hipGetLastError();
 }
-hipError_t launch_bc_compact(const BcDev &b, int *ent_v, unsigned *ent_m, long long *ent_c, int *count,
-                             hipStream_t s) {
-    hipLaunchKernelGGL(k_bc_compact, dim3(bc_grid(b.n, 4096)), dim3(256), 0, s, b, ent_v, ent_m, ent_c, count);
+hipError_t launch_bc_compact(const BcDev &b, int *ent_v, unsigned *ent_m, const int *eoff, long long *ent_c,
+                             int *count, hipStream_t s) {
+    hipLaunchKernelGGL(k_bc_compact, dim3(bc_grid(b.n, 4096)), dim3(256), 0, s, b, ent_v, ent_m, eoff, ent_c, count);
+    return hipGetLastError();
+}
+hipError_t launch_bc_advance(int *eoff, int *count, int *lv, int d, hipStream_t s) {
+    hipLaunchKernelGGL(k_bc_advance, dim3(1), dim3(64), 0, s, eoff, count, lv, d);
     return hipGetLastError();
 }
 hipError_t launch_bc_chunks(const BcDev &b, const int *ent_v, const unsigned *ent_m, int ne, long long *ent_c,
@@ -198,18 +233,26 @@ hipError_t launch_bc_chunks(const BcDev &b, const int *ent_v, const unsigned *en
     hipLaunchKernelGGL(k_bc_chunks, dim3(bc_grid(ne, 4096)), dim3(256), 0, s, b, ent_v, ent_m, ne, ent_c);
     return hipGetLastError();
 }
-hipError_t launch_bc_owner(const long long *cpref, int ne, int *owner, long long cap, int *err, hipStream_t s) {
-    const long long g = ((long long)ne + 3) / 4;  // 4 wavefronts per workgroup, one entry each
-    const int grid = (int)(g < 1 ? 1 : (g > 16384 ? 16384 : g));
-    hipLaunchKernelGGL(k_bc_owner, dim3(grid), dim3(256), 0, s, cpref, ne, owner, cap, err);
+hipError_t launch_bc_owner(const long long *cpref, int ne, const int *ne_dev, int *owner, long long cap, int *err,
+                           hipStream_t s) {
+    // 4 wavefronts per workgroup, one entry each, striding; with the count on the device ne is
+    // only a bound, so the grid is capped lower
+    const long long g = ((long long)ne + 3) / 4;
+    const long long gmax = ne_dev ? 2048 : 16384;
+    const int grid = (int)(g < 1 ? 1 : (g > gmax ? gmax : g));
+    hipLaunchKernelGGL(k_bc_owner, dim3(grid), dim3(256), 0, s, cpref, ne, ne_dev, owner, cap, err);
     return hipGetLastError();
 }
-hipError_t launch_bc_expand(const BcDev &b, bool fwd, const int *ent_v, const unsigned *ent_m,
+hipError_t launch_bc_expand(const BcDev &b, bool fwd, const int *ent_v, const unsigned *ent_m, const int *eoff,
                             const long long *cpref, const int *owner, int ne, long long cap, int d, int num_cus,
                             hipStream_t s) {
     const int grid = num_cus * 16;  // grid-stride over the chunks (their count is on the device)
-    if (fwd) hipLaunchKernelGGL(k_bc_expand<true>, dim3(grid), dim3(256), 0, s, b, ent_v, ent_m, cpref, owner, ne, cap, d);
-    else hipLaunchKernelGGL(k_bc_expand<false>, dim3(grid), dim3(256), 0, s, b, ent_v, ent_m, cpref, owner, ne, cap, d);
+    if (fwd)
+        hipLaunchKernelGGL(k_bc_expand<true>, dim3(grid), dim3(256), 0, s, b, ent_v, ent_m, eoff, cpref, owner, ne,
+                           cap, d);
+    else
+        hipLaunchKernelGGL(k_bc_expand<false>, dim3(grid), dim3(256), 0, s, b, ent_v, ent_m, eoff, cpref, owner, ne,
+                           cap, d);
     return hipGetLastError();
 }
 hipError_t launch_bc_accum(const BcDev &b, const int *ent_v, const unsigned *ent_m, int ne, double *bc,

@@ -132,14 +132,21 @@ struct BcDev {
 hipError_t launch_bc_reset(const BcDev &b, hipStream_t s);
 hipError_t launch_bc_sources(const BcDev &b, const int *srcs, int nsrc, hipStream_t s);
 // Flags -> entries (v, mask) at ent_v/ent_m[*count...] with chunk counts ent_c[*count...].
-hipError_t launch_bc_compact(const BcDev &b, int *ent_v, unsigned *ent_m, long long *ent_c, int *count,
-                             hipStream_t s);
+// Flagged vertices -> entries at ent_v/ent_m + *eoff, counted in *count; chunk counts at
+// ent_c[0 .. count).
+hipError_t launch_bc_compact(const BcDev &b, int *ent_v, unsigned *ent_m, const int *eoff, long long *ent_c,
+                             int *count, hipStream_t s);
+// Level d's first entry and count into lv[2d], lv[2d + 1]; *eoff moves past them, *count = 0.
+hipError_t launch_bc_advance(int *eoff, int *count, int *lv, int d, hipStream_t s);
 hipError_t launch_bc_chunks(const BcDev &b, const int *ent_v, const unsigned *ent_m, int ne, long long *ent_c,
                             hipStream_t s);
-// Chunk owners of ne entries; a chunk total above cap sets *err (nothing is written past cap).
-hipError_t launch_bc_owner(const long long *cpref, int ne, int *owner, long long cap, int *err, hipStream_t s);
-// Expansion of the chunks of ne entries (count cpref[ne], read on the device, at most cap).
-hipError_t launch_bc_expand(const BcDev &b, bool fwd, const int *ent_v, const unsigned *ent_m,
+// Chunk owners of ne entries (*ne_dev when given; ne then bounds it); a chunk total above cap
+// sets *err (nothing is written past cap).
+hipError_t launch_bc_owner(const long long *cpref, int ne, const int *ne_dev, int *owner, long long cap, int *err,
+                           hipStream_t s);
+// Expansion of the chunks of the entries at ent_v + *eoff (eoff null: ent_v); chunk total
+// cpref[ne], read on the device, at most cap.
+hipError_t launch_bc_expand(const BcDev &b, bool fwd, const int *ent_v, const unsigned *ent_m, const int *eoff,
                             const long long *cpref, const int *owner, int ne, long long cap, int d, int num_cus,
                             hipStream_t s);
 hipError_t launch_bc_accum(const BcDev &b, const int *ent_v, const unsigned *ent_m, int ne, double *bc,
