@@ -43,6 +43,18 @@ def test_pagerank_matches_oracle(npa, ctx, orc, kind):
     G.free()
 
 
+def test_pagerank_without_edges(npa, ctx, orc):
+    """Every vertex dangles: the dangling term alone keeps every rank at 1000 / n, so the first
+    iteration changes nothing and the run stops after it."""
+    n = 9
+    G = npa.Graph.from_csr(ctx, np.zeros(n + 1, np.int32), np.zeros(0, np.int32), np.zeros(0, np.int32))
+    got, st = ctx.pagerank(G)
+    assert np.allclose(got, 1000.0 / n, rtol=1e-6)
+    ref, it = orc.pagerank(np.zeros(n + 1, np.int32), np.zeros(0, np.int32))
+    assert st.iterations == it == 1
+    G.free()
+
+
 def test_pagerank_iteration_cap(npa, ctx):
     G = npa.Graph.rmat(ctx, 10, 8, wmin=1, wmax=1, seed=75)
     with pytest.raises(npa.NpError) as e:

@@ -40,6 +40,22 @@ def test_spmv_matches_oracle(npa, ctx, orc, kind):
     G.free()
 
 
+def test_spmv_without_entries_and_with_empty_rows(npa, ctx, orc):
+    """No entries at all (y = 0), then a matrix whose rows are empty but for one."""
+    G = npa.Graph.from_csr(ctx, np.zeros(8, np.int32), np.zeros(0, np.int32), np.zeros(0, np.int32))
+    y, _ = ctx.spmv(G, np.zeros(0), np.arange(7, dtype=np.float64))
+    assert np.array_equal(y, np.zeros(7))
+    G.free()
+    row = np.array([0, 0, 0, 5, 5, 5], np.int32)
+    col = np.array([4, 0, 4, 1, 2], np.int32)
+    G = npa.Graph.from_csr(ctx, row, col, np.ones(5, np.int32))
+    data, x = orc.spmv_reference_values(5, 5)
+    y, _ = ctx.spmv(G, data, x)
+    assert np.array_equal(y, orc.spmv(row, col, data, x))
+    assert y[0] == y[1] == y[3] == y[4] == 0.0
+    G.free()
+
+
 def test_gpu_spmv_cli_matches_oracle(tmp_path, orc):
     g = load_golden("d9_rand.npz")
     out = tmp_path / "y.txt"
