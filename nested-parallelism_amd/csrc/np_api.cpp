@@ -814,9 +814,10 @@ int np_bc(np_ctx *c, np_graph *g, const int32_t *srcs, int32_t nsrc, float *bc_o
     HIPCHK(dalloc(&bb.out, (size_t)n));
     HIPCHK(hipHostMalloc((void **)&bb.h_total, 2 * sizeof(long long)));
     // Forward levels go out several at a time with the entry counts kept on the device; the host
-    // reads each run's (first entry, count) pairs once.  A run is as long as the previous source
-    // batch's depth + 1 (depths barely vary on one graph), at most BC_RUN, then 2 at a time;
-    // levels past the last are empty and cost a few small launches.
+    // reads each run's (first entry, count) pairs once.  The first run is as long as the previous
+    // source batch's depth + 1 (depths barely vary on one graph), at most BC_RUN; runs after it
+    // have 2, 4, 8, ... levels (deep graphs), at most BC_RUN; levels past the last are empty and
+    // cost a few small launches.
     constexpr int BC_RUN = 64;
     HIPCHK(dalloc(&bb.ectl, 2));  // first entry of the current level, its entry count
     HIPCHK(dalloc(&bb.lv, 2 * ((size_t)n + BC_RUN + 1)));
@@ -847,7 +848,8 @@ int np_bc(np_ctx *c, np_graph *g, const int32_t *srcs, int32_t nsrc, float *bc_o
         levels.clear();
         long long E = 0;
         HIPCHK(hipMemsetAsync(bb.ectl, 0, 2 * sizeof(int), c->stream));
-        for (int d0 = 0, last = -1, k = run; last < 0; d0 += k, k = 2) {
+        for (int d0 = 0, last = -1, k = run, grow = 2; last < 0;
+             d0 += k, k = grow, grow = std::min(BC_RUN, 2 * grow)) {
             if (d0 > n) { set_error("more than n BFS levels"); return NP_ERR_NOT_CONVERGED; }
             for (int d = d0; d < d0 + k; ++d) {
                 // chunk counts of the level's entries, zero up to n: the scan of n + 1 counts needs
