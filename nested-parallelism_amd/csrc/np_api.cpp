@@ -759,13 +759,12 @@ struct BcBufs {
     int *ectl = nullptr, *lv = nullptr, *h_lv = nullptr;
     unsigned *ent_m = nullptr, *nmask = nullptr;
     double *sig = nullptr, *del = nullptr, *acc = nullptr;
-    long long *ent_c = nullptr, *cpref = nullptr, *h_total = nullptr;
+    long long *cst = nullptr, *h_total = nullptr;
+    unsigned long long *ctot = nullptr;
     float *out = nullptr;
-    void *temp = nullptr;
     ~BcBufs() {
         dfree(lvl); dfree(ent_v); dfree(owner); dfree(srcs); dfree(err); dfree(ent_m); dfree(nmask);
-        dfree(sig); dfree(del); dfree(acc); dfree(ent_c); dfree(cpref); dfree(out); dfree(ectl); dfree(lv);
-        if (temp) (void)hipFree(temp);
+        dfree(sig); dfree(del); dfree(acc); dfree(cst); dfree(ctot); dfree(out); dfree(ectl); dfree(lv);
         if (h_total) (void)hipHostFree(h_total);
         if (h_lv) (void)hipHostFree(h_lv);
     }
@@ -804,8 +803,7 @@ int np_bc(np_ctx *c, np_graph *g, const int32_t *srcs, int32_t nsrc, float *bc_o
     HIPCHK(dalloc(&bb.nmask, (size_t)n));
     HIPCHK(dalloc(&bb.ent_v, (size_t)cap_e));
     HIPCHK(dalloc(&bb.ent_m, (size_t)cap_e));
-    HIPCHK(dalloc(&bb.ent_c, (size_t)n + 1));
-    HIPCHK(dalloc(&bb.cpref, (size_t)n + 1));
+    HIPCHK(dalloc(&bb.cst, (size_t)n + 1));
     HIPCHK(dalloc(&bb.owner, (size_t)cap_owner));
     HIPCHK(dalloc(&bb.err, 1));
     HIPCHK(hipMemsetAsync(bb.err, 0, sizeof(int), c->stream));
@@ -823,21 +821,12 @@ int np_bc(np_ctx *c, np_graph *g, const int32_t *srcs, int32_t nsrc, float *bc_o
     HIPCHK(dalloc(&bb.lv, 2 * ((size_t)n + BC_RUN + 1)));
     HIPCHK(hipHostMalloc((void **)&bb.h_lv, 2 * BC_RUN * sizeof(int)));
     int run = 8;
-    size_t temp_bytes = 0;
-    HIPCHK(bc_scan(nullptr, &temp_bytes, bb.ent_c, bb.cpref, n, c->stream));
-    HIPCHK(hipMalloc(&bb.temp, temp_bytes ? temp_bytes : 1));
+    // chunk counters, one per forward level (ctot[d]) and per backward level (ctot[NB + li]),
+    // zeroed once per source batch
+    const size_t NB = (size_t)n + BC_RUN + 1;
+    HIPCHK(dalloc(&bb.ctot, 2 * NB));
     HIPCHK(hipMemsetAsync(bb.acc, 0, sizeof(double) * (size_t)n, c->stream));
     BcDev b{g->row, g->col, n, ks, bb.lvl, bb.sig, bb.del, bb.nmask};
-    // chunks of the ne entries at ev/em (ent_c holds their counts): prefix and owners, all on the
-    // stream (the expansion reads the chunk total from cpref[ne])
-    auto chunks = [&](int ne, const int *ev, const unsigned *em) -> int {
-        HIPCHK(launch_bc_chunks(b, ev, em, ne, bb.ent_c, c->stream));
-        HIPCHK(hipMemsetAsync(bb.ent_c + ne, 0, sizeof(long long), c->stream));
-        size_t tb = temp_bytes;
-        HIPCHK(bc_scan(bb.temp, &tb, bb.ent_c, bb.cpref, ne, c->stream));
-        HIPCHK(launch_bc_owner(bb.cpref, ne, nullptr, bb.owner, cap_owner, bb.err, c->stream));
-        return NP_OK;
-    };
     HIPCHK(hipEventRecord(c->ev0, c->stream));
     std::vector<std::pair<long long, int>> levels;
     for (size_t b0 = 0; b0 < S.size(); b0 += (size_t)P) {
@@ -848,19 +837,18 @@ int np_bc(np_ctx *c, np_graph *g, const int32_t *srcs, int32_t nsrc, float *bc_o
         levels.clear();
         long long E = 0;
         HIPCHK(hipMemsetAsync(bb.ectl, 0, 2 * sizeof(int), c->stream));
+        HIPCHK(hipMemsetAsync(bb.ctot, 0, 2 * NB * sizeof(unsigned long long), c->stream));
         for (int d0 = 0, last = -1, k = run, grow = 2; last < 0;
              d0 += k, k = grow, grow = std::min(BC_RUN, 2 * grow)) {
             if (d0 > n) { set_error("more than n BFS levels"); return NP_ERR_NOT_CONVERGED; }
             for (int d = d0; d < d0 + k; ++d) {
-                // chunk counts of the level's entries, zero up to n: the scan of n + 1 counts needs
-                // no host-side entry count (the compaction reads all n vertices anyway)
-                HIPCHK(hipMemsetAsync(bb.ent_c, 0, sizeof(long long) * ((size_t)n + 1), c->stream));
-                HIPCHK(launch_bc_compact(b, bb.ent_v, bb.ent_m, bb.ectl, bb.ent_c, bb.ectl + 1, c->stream));
-                size_t tb = temp_bytes;
-                HIPCHK(bc_scan(bb.temp, &tb, bb.ent_c, bb.cpref, n, c->stream));
-                HIPCHK(launch_bc_owner(bb.cpref, n, bb.ectl + 1, bb.owner, cap_owner, bb.err, c->stream));
-                HIPCHK(launch_bc_expand(b, true, bb.ent_v, bb.ent_m, bb.ectl, bb.cpref, bb.owner, n, cap_owner, d,
-                                        c->num_cus, c->stream));
+                // entry count and chunk total stay on the device: compaction (reserving each
+                // entry's chunks), owners, expansion, then the level's record
+                HIPCHK(launch_bc_compact(b, bb.ent_v, bb.ent_m, bb.ectl, bb.cst, bb.ectl + 1, bb.ctot + d, c->stream));
+                HIPCHK(launch_bc_owner(b, bb.ent_v, bb.ent_m, bb.ectl, bb.cst, n, bb.ectl + 1, bb.owner, cap_owner,
+                                       bb.err, c->stream));
+                HIPCHK(launch_bc_expand(b, true, bb.ent_v, bb.ent_m, bb.ectl, bb.cst, bb.owner, bb.ctot + d, cap_owner,
+                                        d, c->num_cus, c->stream));
                 HIPCHK(launch_bc_advance(bb.ectl, bb.ectl + 1, bb.lv, d, c->stream));
             }
             HIPCHK(hipMemcpyAsync(bb.h_lv, bb.lv + 2 * (size_t)d0, 2 * (size_t)k * sizeof(int), hipMemcpyDeviceToHost,
@@ -884,9 +872,11 @@ int np_bc(np_ctx *c, np_graph *g, const int32_t *srcs, int32_t nsrc, float *bc_o
         for (int li = (int)levels.size() - 2; li >= 1; --li) {
             const long long E0 = levels[(size_t)li].first;
             const int ne = levels[(size_t)li].second;
-            int rc = chunks(ne, bb.ent_v + E0, bb.ent_m + E0);
-            if (rc) return rc;
-            HIPCHK(launch_bc_expand(b, false, bb.ent_v + E0, bb.ent_m + E0, nullptr, bb.cpref, bb.owner, ne, cap_owner,
+            unsigned long long *ct = bb.ctot + NB + li;
+            HIPCHK(launch_bc_chunks(b, bb.ent_v + E0, bb.ent_m + E0, ne, bb.cst, ct, c->stream));
+            HIPCHK(launch_bc_owner(b, bb.ent_v + E0, bb.ent_m + E0, nullptr, bb.cst, ne, nullptr, bb.owner, cap_owner,
+                                   bb.err, c->stream));
+            HIPCHK(launch_bc_expand(b, false, bb.ent_v + E0, bb.ent_m + E0, nullptr, bb.cst, bb.owner, ct, cap_owner,
                                     li, c->num_cus, c->stream));
             HIPCHK(launch_bc_accum(b, bb.ent_v + E0, bb.ent_m + E0, ne, bb.acc, c->stream));
         }

@@ -19,17 +19,40 @@
 //
 // Work split: entry o has deg(v) x P' items, P' = popc(mask) rounded up to a power of two, item
 // r = (edge r / P', source slot r % P').  Items are cut into 64-item chunks (one wavefront each);
-// an owner array maps every chunk to its entry.  Chunk starts are multiples of 64, so a lane's
+// each entry's chunks are reserved as one contiguous range of the level's chunk counter (one
+// atomic per wavefront of entries, no prefix sum) and an owner array maps every chunk to its
+// entry.  Chunk starts are multiples of 64, so a lane's
 // source slot is lane % P' and the lanes of one source are lane, lane ^ P', ...: the backward
 // sweep sums them with xor-shuffles before one atomicAdd per source.
 #include <hip/hip_runtime.h>
-#include <cstring>
-#include <rocprim/device/device_scan.hpp>
 #include "np_internal.h"
 
 namespace np {
 
 __device__ __forceinline__ int bc_lane() { return threadIdx.x & 63; }
+
+// chunks of the entry (v, m): ceil(deg(v) x P' / 64)
+__device__ __forceinline__ long long bc_nchunks(const BcDev &b, int v, unsigned m) {
+    const int p = __popc(m);
+    const int lp = p <= 1 ? 0 : 32 - __clz(p - 1);
+    return (((long long)(b.row[v + 1] - b.row[v]) << lp) + 63) >> 6;
+}
+
+// Every lane of the wave calls it: k chunks for this lane's entry (0 for none) are reserved on
+// *ctot; returns the first chunk of the lane's range.
+__device__ __forceinline__ long long bc_reserve(long long k, unsigned long long *ctot) {
+    const int lane = bc_lane();
+    long long incl = k;
+    for (int o = 1; o < 64; o <<= 1) {
+        const long long t = __shfl_up(incl, o);
+        if (lane >= o) incl += t;
+    }
+    const long long tot = __shfl(incl, 63);
+    long long base = 0;
+    if (lane == 0 && tot) base = (long long)atomicAdd(ctot, (unsigned long long)tot);
+    base = __shfl(base, 0);
+    return base + incl - k;
+}
 
 // k-th set bit of m (k < popc(m))
 __device__ __forceinline__ int bc_nth_bit(unsigned m, int k) {
@@ -59,9 +82,11 @@ __global__ void k_bc_sources(BcDev b, const int *srcs, int nsrc) {
 }
 
 // Flagged vertices -> entries (v, mask) at ent_v/ent_m + *eoff + (position counted in *count),
-// and their chunk counts at ent_c[position].  Entries of all levels fit in n x sources: every
-// (vertex, source) pair is in one level's entry and every entry holds at least one.
-__global__ void k_bc_compact(BcDev b, int *ent_v, unsigned *ent_m, const int *eoff, long long *ent_c, int *count) {
+// and the first chunk of each at cst[position] (chunks reserved on *ctot).  Entries of all
+// levels fit in n x sources: every (vertex, source) pair is in one level's entry and every
+// entry holds at least one.
+__global__ void k_bc_compact(BcDev b, int *ent_v, unsigned *ent_m, const int *eoff, long long *cst, int *count,
+                             unsigned long long *ctot) {
     const int E = *eoff;
     for (long long base = (long long)blockIdx.x * blockDim.x; base < b.n; base += (long long)gridDim.x * blockDim.x) {
         const long long v = base + threadIdx.x;
@@ -73,56 +98,62 @@ __global__ void k_bc_compact(BcDev b, int *ent_v, unsigned *ent_m, const int *eo
         int pos0 = 0;
         if (lane == leader) pos0 = atomicAdd(count, __popcll(want));
         pos0 = __shfl(pos0, leader);
+        const long long first = bc_reserve(m ? bc_nchunks(b, (int)v, m) : 0, ctot);
         if (m) {
             const int pos = pos0 + __popcll(want & ((1ull << lane) - 1));
             b.nmask[v] = 0u;
-            const int p = __popc(m);
-            const int lp = p <= 1 ? 0 : 32 - __clz(p - 1);
-            const long long items = (long long)(b.row[v + 1] - b.row[v]) << lp;
             ent_v[E + pos] = (int)v;
             ent_m[E + pos] = m;
-            ent_c[pos] = (items + 63) >> 6;
+            cst[pos] = first;
         }
     }
 }
 
-// Chunk counts of ne entries (the backward sweep replays entries whose counts are gone).
-__global__ void k_bc_chunks(BcDev b, const int *ent_v, const unsigned *ent_m, int ne, long long *ent_c) {
-    for (long long o = (long long)blockIdx.x * blockDim.x + threadIdx.x; o < ne; o += (long long)gridDim.x * blockDim.x) {
-        const int v = ent_v[o];
-        const int p = __popc(ent_m[o]);
-        const int lp = p <= 1 ? 0 : 32 - __clz(p - 1);
-        ent_c[o] = (((long long)(b.row[v + 1] - b.row[v]) << lp) + 63) >> 6;
+// Chunk ranges of ne entries replayed by the backward sweep: first chunk of each at cst[o].
+__global__ void k_bc_chunks(BcDev b, const int *ent_v, const unsigned *ent_m, int ne, long long *cst,
+                            unsigned long long *ctot) {
+    for (long long base = (long long)blockIdx.x * blockDim.x; base < ne; base += (long long)gridDim.x * blockDim.x) {
+        const long long o = base + threadIdx.x;  // the trip count is uniform over the workgroup
+        const long long first = bc_reserve(o < ne ? bc_nchunks(b, ent_v[o], ent_m[o]) : 0, ctot);
+        if (o < ne) cst[o] = first;
     }
 }
 
-// owner[c] = o for every chunk c of entry o (cpref: exclusive prefix of chunk counts, ne + 1;
-// ne from *ne_dev when given), one wavefront per entry.  A level has at most m/2 + ne chunks
-// (deg x 32 / 64 + 1 per entry); a total above cap sets *err and writes nothing past it.
-__global__ void k_bc_owner(const long long *cpref, int ne, const int *ne_dev, int *owner, long long cap, int *err) {
+// owner[c] = o for every chunk c of entry o (entries at ent_v/ent_m + *eoff, eoff null: at
+// ent_v; first chunk cst[o]; ne from *ne_dev when given), one wavefront per entry.  A level has
+// at most m/2 + ne chunks (deg x 32 / 64 + 1 per entry); a range past cap sets *err and
+// nothing is written past it.
+__global__ void k_bc_owner(BcDev b, const int *ent_v, const unsigned *ent_m, const int *eoff, const long long *cst,
+                           int ne, const int *ne_dev, int *owner, long long cap, int *err) {
     if (ne_dev) ne = *ne_dev;
-    const int lane = bc_lane();
-    const long long nw = (long long)gridDim.x * (blockDim.x >> 6);
-    if (blockIdx.x == 0 && threadIdx.x == 0 && cpref[ne] > cap) *err = 1;
-    for (long long o = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); o < ne; o += nw) {
-        const long long c1 = cpref[o + 1] < cap ? cpref[o + 1] : cap;
-        for (long long c = cpref[o] + lane; c < c1; c += 64) owner[c] = (int)o;
-    }
-}
-
-// One 64-item chunk per wavefront.  FWD: level d -> d + 1 with path counts; else the
-// dependencies of level d from level d + 1.  The level's entries start at ent_v + *eoff (eoff
-// null: at ent_v); cpref[ne] is the chunk total.
-template <bool FWD>
-__global__ __launch_bounds__(256) void k_bc_expand(BcDev b, const int *ent_v, const unsigned *ent_m,
-                                                   const int *eoff, const long long *cpref, const int *owner,
-                                                   int ne, long long cap, int d) {
     if (eoff) {
         ent_v += *eoff;
         ent_m += *eoff;
     }
     const int lane = bc_lane();
-    const long long nchunks = cpref[ne] < cap ? cpref[ne] : cap;
+    const long long nw = (long long)gridDim.x * (blockDim.x >> 6);
+    for (long long o = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); o < ne; o += nw) {
+        const long long c0 = cst[o], c1 = c0 + bc_nchunks(b, ent_v[o], ent_m[o]);
+        if (c1 > cap && lane == 0) *err = 1;
+        const long long ce = c1 < cap ? c1 : cap;
+        for (long long c = c0 + lane; c < ce; c += 64) owner[c] = (int)o;
+    }
+}
+
+// One 64-item chunk per wavefront.  FWD: level d -> d + 1 with path counts; else the
+// dependencies of level d from level d + 1.  The level's entries start at ent_v + *eoff (eoff
+// null: at ent_v); entry o's chunks start at cst[o]; *ctot is the chunk total.
+template <bool FWD>
+__global__ __launch_bounds__(256) void k_bc_expand(BcDev b, const int *ent_v, const unsigned *ent_m,
+                                                   const int *eoff, const long long *cst, const int *owner,
+                                                   const unsigned long long *ctot, long long cap, int d) {
+    if (eoff) {
+        ent_v += *eoff;
+        ent_m += *eoff;
+    }
+    const int lane = bc_lane();
+    const long long tot = (long long)*ctot;
+    const long long nchunks = tot < cap ? tot : cap;
     const long long nw = (long long)gridDim.x * (blockDim.x >> 6);
     for (long long c = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); c < nchunks; c += nw) {
         const int o = owner[c];
@@ -132,7 +163,7 @@ __global__ __launch_bounds__(256) void k_bc_expand(BcDev b, const int *ent_v, co
         const int lp = p <= 1 ? 0 : 32 - __clz(p - 1);
         const int e0 = b.row[v];
         const long long items = (long long)(b.row[v + 1] - e0) << lp;
-        const long long r = ((c - cpref[o]) << 6) + lane;
+        const long long r = ((c - cst[o]) << 6) + lane;
         const int j = lane & ((1 << lp) - 1);
         const bool live = r < items && j < p;
         const int src = live ? bc_nth_bit(m, j) : 0;
@@ -219,39 +250,42 @@ hipError_t launch_bc_sources(const BcDev &b, const int *srcs, int nsrc, hipStrea
     hipLaunchKernelGGL(k_bc_sources, dim3(1), dim3(64), 0, s, b, srcs, nsrc);
     return hipGetLastError();
 }
-hipError_t launch_bc_compact(const BcDev &b, int *ent_v, unsigned *ent_m, const int *eoff, long long *ent_c,
-                             int *count, hipStream_t s) {
-    hipLaunchKernelGGL(k_bc_compact, dim3(bc_grid(b.n, 4096)), dim3(256), 0, s, b, ent_v, ent_m, eoff, ent_c, count);
+hipError_t launch_bc_compact(const BcDev &b, int *ent_v, unsigned *ent_m, const int *eoff, long long *cst,
+                             int *count, unsigned long long *ctot, hipStream_t s) {
+    hipLaunchKernelGGL(k_bc_compact, dim3(bc_grid(b.n, 4096)), dim3(256), 0, s, b, ent_v, ent_m, eoff, cst, count,
+                       ctot);
     return hipGetLastError();
 }
 hipError_t launch_bc_advance(int *eoff, int *count, int *lv, int d, hipStream_t s) {
     hipLaunchKernelGGL(k_bc_advance, dim3(1), dim3(64), 0, s, eoff, count, lv, d);
     return hipGetLastError();
 }
-hipError_t launch_bc_chunks(const BcDev &b, const int *ent_v, const unsigned *ent_m, int ne, long long *ent_c,
-                            hipStream_t s) {
-    hipLaunchKernelGGL(k_bc_chunks, dim3(bc_grid(ne, 4096)), dim3(256), 0, s, b, ent_v, ent_m, ne, ent_c);
+hipError_t launch_bc_chunks(const BcDev &b, const int *ent_v, const unsigned *ent_m, int ne, long long *cst,
+                            unsigned long long *ctot, hipStream_t s) {
+    hipLaunchKernelGGL(k_bc_chunks, dim3(bc_grid(ne, 4096)), dim3(256), 0, s, b, ent_v, ent_m, ne, cst, ctot);
     return hipGetLastError();
 }
-hipError_t launch_bc_owner(const long long *cpref, int ne, const int *ne_dev, int *owner, long long cap, int *err,
+hipError_t launch_bc_owner(const BcDev &b, const int *ent_v, const unsigned *ent_m, const int *eoff,
+                           const long long *cst, int ne, const int *ne_dev, int *owner, long long cap, int *err,
                            hipStream_t s) {
     // 4 wavefronts per workgroup, one entry each, striding; with the count on the device ne is
     // only a bound, so the grid is capped lower
     const long long g = ((long long)ne + 3) / 4;
     const long long gmax = ne_dev ? 2048 : 16384;
     const int grid = (int)(g < 1 ? 1 : (g > gmax ? gmax : g));
-    hipLaunchKernelGGL(k_bc_owner, dim3(grid), dim3(256), 0, s, cpref, ne, ne_dev, owner, cap, err);
+    hipLaunchKernelGGL(k_bc_owner, dim3(grid), dim3(256), 0, s, b, ent_v, ent_m, eoff, cst, ne, ne_dev, owner, cap,
+                       err);
     return hipGetLastError();
 }
 hipError_t launch_bc_expand(const BcDev &b, bool fwd, const int *ent_v, const unsigned *ent_m, const int *eoff,
-                            const long long *cpref, const int *owner, int ne, long long cap, int d, int num_cus,
-                            hipStream_t s) {
+                            const long long *cst, const int *owner, const unsigned long long *ctot, long long cap,
+                            int d, int num_cus, hipStream_t s) {
     const int grid = num_cus * 16;  // grid-stride over the chunks (their count is on the device)
     if (fwd)
-        hipLaunchKernelGGL(k_bc_expand<true>, dim3(grid), dim3(256), 0, s, b, ent_v, ent_m, eoff, cpref, owner, ne,
+        hipLaunchKernelGGL(k_bc_expand<true>, dim3(grid), dim3(256), 0, s, b, ent_v, ent_m, eoff, cst, owner, ctot,
                            cap, d);
     else
-        hipLaunchKernelGGL(k_bc_expand<false>, dim3(grid), dim3(256), 0, s, b, ent_v, ent_m, eoff, cpref, owner, ne,
+        hipLaunchKernelGGL(k_bc_expand<false>, dim3(grid), dim3(256), 0, s, b, ent_v, ent_m, eoff, cst, owner, ctot,
                            cap, d);
     return hipGetLastError();
 }
@@ -263,11 +297,6 @@ hipError_t launch_bc_accum(const BcDev &b, const int *ent_v, const unsigned *ent
 hipError_t launch_bc_to_float(const double *bc, int n, float *out, hipStream_t s) {
     hipLaunchKernelGGL(k_bc_to_float, dim3(bc_grid(n, 4096)), dim3(256), 0, s, bc, n, out);
     return hipGetLastError();
-}
-
-// Exclusive prefix of n chunk counts into out[0..n] (out[n] = total); temp may be null to query.
-hipError_t bc_scan(void *temp, size_t *temp_bytes, const long long *in, long long *out, int n, hipStream_t s) {
-    return rocprim::exclusive_scan(temp, *temp_bytes, in, out, 0ll, (size_t)n + 1, rocprim::plus<long long>(), s);
 }
 
 }  // namespace np

@@ -132,29 +132,28 @@ struct BcDev {
 hipError_t launch_bc_reset(const BcDev &b, hipStream_t s);
 hipError_t launch_bc_sources(const BcDev &b, const int *srcs, int nsrc, hipStream_t s);
 // Flags -> entries (v, mask) at ent_v/ent_m[*count...] with chunk counts ent_c[*count...].
-// Flagged vertices -> entries at ent_v/ent_m + *eoff, counted in *count; chunk counts at
-// ent_c[0 .. count).
-hipError_t launch_bc_compact(const BcDev &b, int *ent_v, unsigned *ent_m, const int *eoff, long long *ent_c,
-                             int *count, hipStream_t s);
+// Flagged vertices -> entries at ent_v/ent_m + *eoff, counted in *count; the first chunk of
+// each at cst[0 .. count), chunks reserved on *ctot (zero before the level).
+hipError_t launch_bc_compact(const BcDev &b, int *ent_v, unsigned *ent_m, const int *eoff, long long *cst,
+                             int *count, unsigned long long *ctot, hipStream_t s);
 // Level d's first entry and count into lv[2d], lv[2d + 1]; *eoff moves past them, *count = 0.
 hipError_t launch_bc_advance(int *eoff, int *count, int *lv, int d, hipStream_t s);
-hipError_t launch_bc_chunks(const BcDev &b, const int *ent_v, const unsigned *ent_m, int ne, long long *ent_c,
-                            hipStream_t s);
-// Chunk owners of ne entries (*ne_dev when given; ne then bounds it); a chunk total above cap
-// sets *err (nothing is written past cap).
-hipError_t launch_bc_owner(const long long *cpref, int ne, const int *ne_dev, int *owner, long long cap, int *err,
+// Chunk ranges of ne entries (first chunk at cst[o]) reserved on *ctot (zero before).
+hipError_t launch_bc_chunks(const BcDev &b, const int *ent_v, const unsigned *ent_m, int ne, long long *cst,
+                            unsigned long long *ctot, hipStream_t s);
+// Chunk owners of ne entries at ent_v/ent_m + *eoff (*ne_dev when given; ne then bounds it); a
+// range past cap sets *err (nothing is written past cap).
+hipError_t launch_bc_owner(const BcDev &b, const int *ent_v, const unsigned *ent_m, const int *eoff,
+                           const long long *cst, int ne, const int *ne_dev, int *owner, long long cap, int *err,
                            hipStream_t s);
 // Expansion of the chunks of the entries at ent_v + *eoff (eoff null: ent_v); chunk total
-// cpref[ne], read on the device, at most cap.
+// *ctot, read on the device, at most cap.
 hipError_t launch_bc_expand(const BcDev &b, bool fwd, const int *ent_v, const unsigned *ent_m, const int *eoff,
-                            const long long *cpref, const int *owner, int ne, long long cap, int d, int num_cus,
-                            hipStream_t s);
+                            const long long *cst, const int *owner, const unsigned long long *ctot, long long cap,
+                            int d, int num_cus, hipStream_t s);
 hipError_t launch_bc_accum(const BcDev &b, const int *ent_v, const unsigned *ent_m, int ne, double *bc,
                            hipStream_t s);
 hipError_t launch_bc_to_float(const double *bc, int n, float *out, hipStream_t s);
-// Exclusive prefix of in[0..n] into out[0..n] (in[n] must be 0, so out[n] = total); temp = null
-// queries *temp_bytes.
-hipError_t bc_scan(void *temp, size_t *temp_bytes, const long long *in, long long *out, int n, hipStream_t s);
 
 // ---- color.hip: graph coloring (longest decreasing-id path labels) -----------------------
 hipError_t launch_color_count(const int *row, const int *col, int n, int *dep, long long *ncnt, hipStream_t s);
