@@ -1096,8 +1096,10 @@ int np_pagerank(np_ctx *c, np_graph *g, int32_t max_iters, float *rank_out, np_s
     int end_bit = 1;  // key bits covering vertex ids 0..n-1
     while (end_bit < 31 && (1ll << end_bit) < n) ++end_bit;
     const unsigned *tkey_in = reinterpret_cast<const unsigned *>(g->col);
-    size_t temp_bytes = 0;
+    size_t temp_bytes = 0, scan_bytes = 0;
     HIPCHK(pr_sort(nullptr, &temp_bytes, tkey_in, nullptr, nullptr, nullptr, m, end_bit, c->stream));
+    HIPCHK(pr_sources(nullptr, &scan_bytes, g->row, n, m, nullptr, nullptr, c->stream));
+    temp_bytes = std::max(temp_bytes, scan_bytes);
     PrBufs pb{};
     Carve cv;
     pr_carve(cv, pb, n, m, temp_bytes);
@@ -1107,7 +1109,8 @@ int np_pagerank(np_ctx *c, np_graph *g, int32_t max_iters, float *rank_out, np_s
     HIPCHK(hipEventRecord(c->ev0, c->stream));
     HIPCHK(bins_clear(pb.rb, c->stream));
     if (m > 0) {
-        HIPCHK(launch_pr_sources(g->row, n, pb.src, c->stream));
+        size_t sb = scan_bytes;
+        HIPCHK(pr_sources(pb.temp, &sb, g->row, n, m, pb.pcol, pb.src, c->stream));  // pcol: scratch until the sort
         size_t tb = temp_bytes;
         HIPCHK(pr_sort(pb.temp, &tb, tkey_in, pb.tkey, pb.src, pb.pcol, m, end_bit, c->stream));
     }

@@ -182,7 +182,10 @@ struct RowBins {
 };
 long long pr_chunk_cap(long long m);
 // In-edge CSR: src[e] = source of edge e; sort (col, src) pairs; offsets from the sorted targets.
-hipError_t launch_pr_sources(const int *row, int n, int *src, hipStream_t s);
+// src[e] = source of edge e for the m edges of the CSR row (heads: m ints of scratch; temp null
+// queries *temp_bytes).
+hipError_t pr_sources(void *temp, size_t *temp_bytes, const int *row, int n, long long m, int *heads, int *src,
+                      hipStream_t s);
 hipError_t pr_sort(void *temp, size_t *temp_bytes, const unsigned *key_in, unsigned *key_out, const int *val_in,
                    int *val_out, long long m, int end_bit, hipStream_t s);
 hipError_t launch_pr_offsets(const unsigned *key, long long m, int n, long long *irow, hipStream_t s);
