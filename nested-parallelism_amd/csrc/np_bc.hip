@@ -207,17 +207,26 @@ __global__ __launch_bounds__(256) void k_bc_expand(BcDev b, const int *ent_v, co
 // factor the level above needs from it (one gather there instead of two).  Each (vertex,
 // source) slot is in exactly one level's entries, so it is transformed once, after its own
 // level's sweep; the deepest level (delta 0) is passed through here too.
+// Half a wavefront per entry, lane j on the j-th source of its mask: the transforms run in
+// parallel and the entry's deltas are summed with shuffles.
 __global__ void k_bc_accum(BcDev b, const int *ent_v, const unsigned *ent_m, int ne, double *bc) {
-    for (long long o = (long long)blockIdx.x * blockDim.x + threadIdx.x; o < ne; o += (long long)gridDim.x * blockDim.x) {
-        const int v = ent_v[o];
-        double s = 0.0;
-        for (unsigned m = ent_m[o]; m; m &= m - 1) {
-            const long long k = ((long long)v << b.ks) + (__ffs(m) - 1);
-            const double dl = b.del[k];
-            s += dl;
-            b.sig[k] = (1.0 + dl) / b.sig[k];
+    const int j = threadIdx.x & 31;
+    const long long per = (long long)gridDim.x * (blockDim.x / 32);
+    const long long nn = ((long long)ne + 1) / 2 * 2;  // whole waves: both halves make the same trips
+    for (long long o = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / 32; o < nn; o += per) {
+        double dl = 0.0;
+        int v = 0;
+        if (o < ne) {
+            v = ent_v[o];
+            const unsigned m = ent_m[o];
+            if (j < __popc(m)) {
+                const long long k = ((long long)v << b.ks) + bc_nth_bit(m, j);
+                dl = b.del[k];
+                b.sig[k] = (1.0 + dl) / b.sig[k];
+            }
         }
-        bc[v] += s;  // one entry per vertex per level
+        for (int off = 16; off > 0; off >>= 1) dl += __shfl_xor(dl, off);
+        if (o < ne && j == 0) bc[v] += dl;  // one entry per vertex per level
     }
 }
 
@@ -291,7 +300,7 @@ hipError_t launch_bc_expand(const BcDev &b, bool fwd, const int *ent_v, const un
 }
 hipError_t launch_bc_accum(const BcDev &b, const int *ent_v, const unsigned *ent_m, int ne, double *bc,
                            hipStream_t s) {
-    hipLaunchKernelGGL(k_bc_accum, dim3(bc_grid(ne, 4096)), dim3(256), 0, s, b, ent_v, ent_m, ne, bc);
+    hipLaunchKernelGGL(k_bc_accum, dim3(bc_grid((long long)ne * 32, 4096)), dim3(256), 0, s, b, ent_v, ent_m, ne, bc);
     return hipGetLastError();
 }
 hipError_t launch_bc_to_float(const double *bc, int n, float *out, hipStream_t s) {
