@@ -17,8 +17,8 @@ def _graph(npa, ctx, row, col):
     return npa.Graph.from_csr(ctx, row, col, np.ones(len(col), np.int32))
 
 
-def test_bc_directed_path_closed_form(npa, ctx):
-    n = 40
+@pytest.mark.parametrize("n", [40, 300])   # 300: depth 299, forward levels in many growing runs
+def test_bc_directed_path_closed_form(npa, ctx, n):
     row = np.minimum(np.arange(n + 1), n - 1).astype(np.int32)
     col = np.arange(1, n, dtype=np.int32)
     G = _graph(npa, ctx, row, col)
