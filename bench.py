@@ -334,7 +334,7 @@ def pr_bench(npa, ctx, args):
                                "permuted; one step = one np_pagerank to convergence", "n": int(G.n), "m": int(G.m),
                    "iterations": it, "device_ms": round(dev_ms, 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic("pagerank"),
                      "kernel": "k_pr_contrib+k_pr_pull_light+k_pr_pull_wave+k_pr_pull_huge (host loop)",
                      "bytes_per_launch": b_alg},
         "cpu_baseline": cpu, "exact": exact,
@@ -344,6 +344,15 @@ def pr_bench(npa, ctx, args):
 
 
 SPMV_METRIC = "SpMV GFLOP/s + %HBM peak @R-MAT s22 (float64), exact"
+
+
+def measured_traffic(config):
+    """HBM bytes per launch measured with rocprofv3 counters (profiles/traffic_<config>.json), or None."""
+    tpath = os.path.join(ROOT, "profiles", "traffic_%s.json" % config)
+    try:
+        return json.load(open(tpath)).get(config)
+    except (ValueError, OSError):
+        return None
 
 
 def spmv_bench(npa, ctx, args):
@@ -384,7 +393,7 @@ def spmv_bench(npa, ctx, args):
                                "the reference's values; one step = one product", "n": int(G.n), "m": int(G.m),
                    "wall_ms_with_host_copies": round(wall_ms, 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic("spmv"),
                      "kernel": "k_pr_bins+k_spmv_light+k_spmv_wave+k_spmv_huge", "bytes_per_launch": b_alg},
         "cpu_baseline": cpu, "exact": exact,
     }
