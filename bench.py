@@ -335,7 +335,8 @@ def pr_bench(npa, ctx, args):
                    "iterations": it, "device_ms": round(dev_ms, 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic("pagerank"),
-                     "kernel": "k_pr_contrib+k_pr_pull_light+k_pr_pull_wave+k_pr_pull_huge (host loop)",
+                     "kernel": "k_pr_gate+k_pr_contrib+k_pr_pull_light+k_pr_pull_wave+k_pr_pull_huge "
+                               "(8 iterations per readback)",
                      "bytes_per_launch": b_alg},
         "cpu_baseline": cpu, "exact": exact,
     }
