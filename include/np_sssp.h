@@ -326,12 +326,16 @@ int np_color(np_ctx *ctx, np_graph *g, int32_t method, int32_t *color_out, np_st
  *   new[v] = 0.5 * sum_{p -> v} rank[p]/outdeg[p] + 0.5 * 1000/n + 0.5 * (rank of dangling)/n
  * until no rank changes by more than 1e-13 (damping, total and epsilon of the reference), in
  * float64; rank_out[n] receives float values (graph.rankArray, FLOAT_T = float).  max_iters = 0:
- * no cap beyond 100000 (NP_ERR_NOT_CONVERGED past it).  stats.iterations = iterations run. */
+ * no cap beyond 100000 (NP_ERR_NOT_CONVERGED past it).  stats.iterations = iterations run.
+ * Its device buffers (in-edge CSR, row bins, ranks) come from a workspace the context keeps
+ * and grows to the largest call (shared with np_spmv, freed by np_destroy), so repeated calls
+ * allocate nothing; calls on one context must not overlap (one context per thread). */
 int np_pagerank(np_ctx *ctx, np_graph *g, int32_t max_iters, float *rank_out, np_stats *stats);
 
 /* ---- SpMV: SpMV_GPU (apps/SpMV/src/gpu/SpMV_wrapper.cu:314-345, SpMV_kernel.cu:77-92) ------
  * y[v] = sum over the graph's row v of data[e] * x[col[e]] (float64, the reference's FLOAT_T);
- * host arrays data[m], x[n], y[n].  kernel_ms covers the product only (not the copies). */
+ * host arrays data[m], x[n], y[n].  kernel_ms covers the product only (not the copies).  Device
+ * copies of data, x, y and the row bins come from the context's workspace (see np_pagerank). */
 int np_spmv(np_ctx *ctx, np_graph *g, const double *data, const double *x, double *y, np_stats *stats);
 
 /* ---- binary CSR cache -- no counterpart in the reference ----------------------------
