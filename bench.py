@@ -124,8 +124,33 @@ def time_mode(ctx, step, steps, warmup):
     return (time.perf_counter() - t) * 1e3 / steps, launch_ms
 
 
+def host_cpu():
+    """(CPU model string, logical CPUs of the host, CPUs this process may run on)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return model, os.cpu_count() or 1, usable
+
+
 def cpu_leg(W, hosts, finals, modes, m_r, k):
-    """Checker and CPU baseline (the only place the oracle is used)."""
+    """Checker and CPU baseline (the only place the oracle is used).
+
+    1. every instance of every mode: the shortest-path certificate, and modes equal;
+    2. per-source single-core time: the CPU Dijkstra (restated SSSP_CPU) on k instances, one
+       thread, one after another;
+    3. aggregate wall time: every instance of the workload, one Dijkstra per thread on
+       min(#instances, usable CPUs, 16) threads (16 = one GPU's share of the host); every
+       distance compared with the GPU's, bit-exact."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     exact = True
@@ -143,20 +168,36 @@ def cpu_leg(W, hosts, finals, modes, m_r, k):
     log("certificates: %d instances x %d modes in %.1fs: %s" % (len(W.inst), len(modes), time.time() - t, exact))
     k = min(k, len(W.inst))
     t = time.perf_counter()
-    refs = []
     for i in range(k):
         gi, s = W.inst[i]
-        row, col, w = hosts[gi]
-        refs.append(oracle.dijkstra(row, col, w, s)[0])
-    tc = time.perf_counter() - t
-    for i in range(k):
-        if not np.array_equal(refs[i], finals[modes[0]][i]):
+        if not np.array_equal(oracle.dijkstra(*hosts[gi], s)[0], finals[modes[0]][i]):
             exact = False
             log("DIJKSTRA MISMATCH instance %d" % i)
-    cpu = {"value": round(float(sum(m_r[:k])) / tc / 1e9, 5), "unit": "GTEPS", "cores": 1, "kind": "port",
-           "sample": "CPU Dijkstra (restated SSSP_CPU + binary heap) on %d of the %d instances of the same workload, "
-                     "one thread, %.1f s; distances bit-exact vs the GPU" % (k, len(W.inst), tc)}
-    log("cpu baseline: %.5f GTEPS (%.1f s for %d instance(s))" % (cpu["value"], tc, k))
+    t1 = time.perf_counter() - t
+    per_source = t1 / max(1, k)
+    model, ncpu, usable = host_cpu()
+    threads = max(1, min(len(W.inst), usable, 16))
+    t = time.perf_counter()
+    for gi in sorted(set(g for g, _ in W.inst)):
+        ids = [i for i, (g, _) in enumerate(W.inst) if g == gi]
+        dist = oracle.dijkstra_many(*hosts[gi], [W.inst[i][1] for i in ids], threads=threads)
+        for j, i in enumerate(ids):
+            if not np.array_equal(dist[j], finals[modes[0]][i]):
+                exact = False
+                log("DIJKSTRA MISMATCH instance %d (aggregate)" % i)
+        del dist
+    agg = time.perf_counter() - t
+    total_m = float(sum(m_r))
+    cpu = {"value": round(total_m / agg / 1e9, 5), "unit": "GTEPS", "cores": threads, "kind": "port",
+           "nproc": ncpu, "usable_cpus": usable, "cpu_model": model,
+           "per_source_s": round(per_source, 3), "aggregate_s": round(agg, 3),
+           "per_source_gteps_1core": round(float(sum(m_r[:k])) / t1 / 1e9, 5),
+           "sample": "CPU Dijkstra (restated SSSP_CPU + binary heap, common/heap.cpp): all %d instances of the "
+                     "same workload, one Dijkstra per thread on %d threads, %.1f s wall (value = reached edges / "
+                     "that wall time); per source on one core: %.1f s (%d instance(s) in sequence); every distance "
+                     "bit-exact vs the GPU" % (len(W.inst), threads, agg, per_source, k)}
+    log("cpu baseline: %.5f GTEPS aggregate (%d threads, %.1f s), %.1f s per source on one core; %s, nproc %d"
+        % (cpu["value"], threads, agg, per_source, model, ncpu))
     return cpu, exact
 
 
@@ -563,11 +604,17 @@ def main():
         per_launch_bytes = float(b_alg.mean())
     kernel += " (%s drive)" % {0: "persistent", 1: "host-loop", 2: "graph"}[scheds[head].drive]
     achieved = per_launch_bytes / (hr["launch_ms"] * 1e-3) / 1e9
-    traffic = None
+    traffic, traffic_note = None, None
     tpath = os.path.join(ROOT, "profiles", "traffic_%s.json" % args.config)
     if os.path.exists(tpath):
         try:
-            traffic = json.load(open(tpath)).get(head)
+            tj = json.load(open(tpath))
+            rec = tj.get(head)
+            if isinstance(rec, dict):
+                traffic = rec.get("bytes")
+                traffic_note = {"source": "stored, not measured in this run: profiles/traffic_%s.json" % args.config,
+                                "unscaled_bytes": rec.get("unscaled_bytes"), "fetch_scale": rec.get("fetch_scale"),
+                                "write_scale": rec.get("write_scale"), "profile": rec.get("profile")}
         except (ValueError, OSError):
             traffic = None
     base = results.get("plain", {}).get("gteps")
@@ -590,8 +637,8 @@ def main():
                    "n": int(sum(g.n for g in W.graphs)), "m": int(sum(g.m for g in W.graphs)),
                    "reached_edges_per_step": total_m},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kernel,
-                     "bytes_per_launch": per_launch_bytes},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_note": traffic_note,
+                     "kernel": kernel, "bytes_per_launch": per_launch_bytes},
         "cpu_baseline": cpu,
         "exact": exact,
         "modes": {k: {"gteps": round(v["gteps"], 3), "ms_per_step": round(v["ms_per_step"], 3),

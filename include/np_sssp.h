@@ -86,7 +86,8 @@ typedef struct np_sched {
     int32_t graph_chunk; /* NP_DRIVE_GRAPH: iterations per captured chunk (0 = 16) */
     int32_t max_iters;   /* 0 = no cap beyond n+1 */
     int32_t trace;       /* nonzero: record per-iteration trace (np_trace_*) */
-    int32_t options;     /* reserved, 0 */
+    int32_t scheme;      /* NP_SCHEME_*: 0 = the consolidation bins above; others are the
+                            reference's other schedules (config.solution) */
     int32_t delta;       /* scan frontier: priority window.  0 = every flagged vertex is
                             expanded (the reference's unordered frontier); > 0 = only
                             vertices with dist < T are expanded, T rising to
@@ -95,6 +96,38 @@ typedef struct np_sched {
     int32_t reserved[5];
 } np_sched;
 
+
+/* np_sched.scheme: how queued vertices are mapped to workers.  Schemes 1-8 are the
+ * reference's other schedules behind config.solution (apps/sssp/src/gpu/sssp_wrapper.cu:
+ * 553-570, apps/bfs-rec/src/gpu/bfs_rec_wrapper.cu:241-262); a vertex is heavy when its
+ * out-degree is >= 2^log2_t_lo (THRESHOLD 64, apps/sssp/src/gpu/sssp_kernel.cu:7).  They run on
+ * the host-loop drive (a persistent request runs host-loop) or the graph drive, with the scan
+ * frontier and no priority window, on single-source solves (np_sssp, np_bfs); level, the
+ * learned scheduler and log2_t_hi do not apply to them. */
+enum {
+    NP_SCHEME_CONSOLIDATE = 0,   /* degree bins of one launch (solution 6; bfs-rec 3) */
+    NP_SCHEME_THREAD = 1,        /* a lane walks all edges of one vertex (solution 0,
+                                    unorder_threadQueue_kernel, sssp_kernel.cu:34-55) */
+    NP_SCHEME_DUAL_QUEUE = 2,    /* light vertices a lane each, then heavy ones a workgroup each,
+                                    two launches (solution 1, sssp_wrapper.cu:193-293) */
+    NP_SCHEME_LDS_BUFFER = 3,    /* light a lane each; heavy into a per-workgroup LDS buffer that
+                                    the whole workgroup then walks (solution 2, shared delayed
+                                    buffer, sssp_kernel.cu:112-172) */
+    NP_SCHEME_GLOBAL_BUFFER = 4, /* light a lane each; heavy appended to a global buffer that a
+                                    second launch drains, a wavefront per vertex (solution 3,
+                                    sssp_kernel.cu:179-209 + sssp_process_buffer) */
+    NP_SCHEME_WAVE_NESTED = 5,   /* light a lane each; each wavefront then walks its own heavy
+                                    vertices, 64 lanes each (solution 4, a 64-thread nested launch
+                                    per heavy vertex, sssp_kernel.cu:237-286) */
+    NP_SCHEME_WG_NESTED = 6,     /* light a lane each; heavy into a per-workgroup list walked by its
+                                    wavefronts, 64 lanes per vertex (solution 5, one nested launch
+                                    per block, sssp_kernel.cu:289-327) */
+    NP_SCHEME_BFS_FLAT = 7,      /* np_bfs only: level-synchronous scan of every vertex per level
+                                    (bfs-rec solution 0, bfs_kernel_flat, bfs_rec_kernel.cu:41-58) */
+    NP_SCHEME_BFS_HIER = 8       /* np_bfs only: two levels per expansion, lanes over a vertex's
+                                    children and the wavefront over each lowered child's children
+                                    (bfs-rec solution 2, bfs_kernel_dp_hier, bfs_rec_kernel.cu:95-140) */
+};
 
 /* Fills the plain fixed-threshold schedule of the reference at `level` (0,1,2), the
  * persistent drive and the scan frontier. */
@@ -154,6 +187,17 @@ int np_dev_copy(np_ctx *ctx, void *dst, const void *src, uint64_t bytes, int32_t
 /* Waits for all work queued on the context stream. */
 int np_sync(np_ctx *ctx);
 
+/* Checks a host CSR without a device: row[0] = 0, row non-decreasing, row[n] = m, 0 <= col < n
+ * (NP_ERR_ARG otherwise); with weights (w non-null) every weight >= 0 (NP_ERR_ARG: shortest
+ * paths need them) and < NP_INF (NP_ERR_CAPACITY: dist[u] + w must stay in int32, the
+ * reference's int arithmetic, apps/sssp/src/gpu/sssp_kernel.cu:46-52).  *max_w (optional)
+ * receives the largest weight.  np_graph_upload applies it; device-resident graphs
+ * (np_graph_wrap_device, the generators) get the same checks on the device.  Solves whose
+ * distances could reach NP_INF ((n - 1) x max weight >= NP_INF) check after the solve that
+ * none did and fail with NP_ERR_CAPACITY otherwise: the reference reports such a vertex as
+ * unreached.  No counterpart in the reference (it checks nothing). */
+int np_csr_validate(int32_t n, int64_t m, const int32_t *row, const int32_t *col, const int32_t *w,
+                    int32_t *max_w);
 /* Copy a host CSR to the device (prepare_gpu's H2D, sssp_wrapper.cu:100-113). */
 int np_graph_upload(np_ctx *ctx, int32_t n, int64_t m, const int32_t *row, const int32_t *col,
                     const int32_t *w, np_graph **out);

@@ -32,6 +32,9 @@ MLP_NPARAM = 32 * 10 + 32 + 32 * 32 + 32 + 5 * 32 + 5
 LEVEL_WAVE, LEVEL_WORKGROUP, LEVEL_GRID, LEVEL_LEARNED = 0, 1, 2, -1
 DRIVE_PERSISTENT, DRIVE_HOSTLOOP, DRIVE_GRAPH = 0, 1, 2
 FRONTIER_SCAN, FRONTIER_PUSH = 0, 1
+# np_sched.scheme (the reference's config.solution schedules)
+(SCHEME_CONSOLIDATE, SCHEME_THREAD, SCHEME_DUAL_QUEUE, SCHEME_LDS_BUFFER, SCHEME_GLOBAL_BUFFER,
+ SCHEME_WAVE_NESTED, SCHEME_WG_NESTED, SCHEME_BFS_FLAT, SCHEME_BFS_HIER) = range(9)
 
 ERRORS = {0: "NP_OK", -1: "NP_ERR_ARG", -2: "NP_ERR_HIP", -3: "NP_ERR_NOMEM", -4: "NP_ERR_CAPACITY",
           -5: "NP_ERR_NOT_CONVERGED", -6: "NP_ERR_TIMEOUT", -7: "NP_ERR_UNSUPPORTED", -8: "NP_ERR_IO"}
@@ -48,12 +51,12 @@ class Sched(ctypes.Structure):
     _fields_ = [("level", ctypes.c_int32), ("log2_t_lo", ctypes.c_int32), ("log2_t_hi", ctypes.c_int32),
                 ("drive", ctypes.c_int32), ("frontier", ctypes.c_int32), ("wgs_per_cu", ctypes.c_int32),
                 ("graph_chunk", ctypes.c_int32), ("max_iters", ctypes.c_int32), ("trace", ctypes.c_int32),
-                ("options", ctypes.c_int32), ("delta", ctypes.c_int32), ("reserved", ctypes.c_int32 * 5)]
+                ("scheme", ctypes.c_int32), ("delta", ctypes.c_int32), ("reserved", ctypes.c_int32 * 5)]
 
     def __init__(self, level=1, log2_t_lo=6, log2_t_hi=6, drive=DRIVE_PERSISTENT, frontier=FRONTIER_SCAN,
-                 wgs_per_cu=0, graph_chunk=0, max_iters=0, trace=0, options=0, delta=0):
+                 wgs_per_cu=0, graph_chunk=0, max_iters=0, trace=0, scheme=0, delta=0):
         super().__init__(level, log2_t_lo, log2_t_hi, drive, frontier, wgs_per_cu, graph_chunk, max_iters,
-                         trace, options, delta)
+                         trace, scheme, delta)
 
     def copy(self, **kw):
         d = {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
@@ -101,6 +104,7 @@ _SIGS = [
     ("np_dev_free", ctypes.c_int, [_vp, _vp]),
     ("np_dev_copy", ctypes.c_int, [_vp, _vp, _vp, ctypes.c_uint64, ctypes.c_int32]),
     ("np_sync", ctypes.c_int, [_vp]),
+    ("np_csr_validate", ctypes.c_int, [ctypes.c_int32, ctypes.c_int64, _vp, _vp, _vp, _i32p]),
     ("np_graph_upload", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int64, _vp, _vp, _vp, ctypes.POINTER(_vp)]),
     ("np_graph_wrap_device", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int64, _vp, _vp, _vp,
                                             ctypes.POINTER(_vp)]),
@@ -227,6 +231,16 @@ def default_sched(level=1):
 # ---------------------------------------------------------------------------------------
 # host-only helpers (no device needed)
 # ---------------------------------------------------------------------------------------
+def csr_validate(row, col, w=None):
+    """np_csr_validate: raises NpError (NP_ERR_ARG for a malformed CSR or a negative weight,
+    NP_ERR_CAPACITY for a weight >= INF); returns the largest weight."""
+    row, col = _i32(row), _i32(col)
+    w = None if w is None else _i32(w)
+    mx = ctypes.c_int32()
+    _check(load().np_csr_validate(len(row) - 1, len(col), _ptr(row), _ptr(col), _ptr(w), ctypes.byref(mx)))
+    return mx.value
+
+
 def read_graph(fmt, path):
     """readInputDIMACS9/10/SLNDC + convertCSR (fmt 0/1/2; 3 = the binary CSR cache).
     Returns (row, col, w, source)."""

@@ -76,6 +76,7 @@ struct Space {
     int src_cap = 0;
     unsigned *mflag = nullptr;    // source-packed: pending sources per vertex
     unsigned *qmask[2] = {nullptr, nullptr};
+    int *sbuf = nullptr, *sbcnt = nullptr;  // NP_SCHEME_GLOBAL_BUFFER (allocated on first use)
     // a batch part run concurrently with the others: its own control block, stream, event
     Ctrl *pctrl = nullptr;
     Ctrl *h_pctrl = nullptr;
@@ -88,6 +89,10 @@ struct Space {
     DevState gst{};
     int gchunk = 0, gwgs = 0;
     bool gbatch = false, gunit = false;
+    // largest edge weight of the graph(s) solved in this space, and whether a distance could
+    // reach INF: (largest vertex count - 1) x max_w >= INF (checked after every solve)
+    int max_w = 0;
+    bool guard = false;
 
     void release() {
         dfree(dist); dfree(bm[0]); dfree(bm[1]); dfree(queue[0]); dfree(queue[1]);
@@ -96,6 +101,7 @@ struct Space {
         cls = nullptr;
         dfree(lut); dfree(inst); dfree(src_slots);
         dfree(mflag); dfree(qmask[0]); dfree(qmask[1]);
+        dfree(sbuf); dfree(sbcnt);
         dfree(pctrl);
         if (h_pctrl) (void)hipHostFree(h_pctrl);
         if (pstream) (void)hipStreamDestroy(pstream);
@@ -138,7 +144,20 @@ struct np_ctx {
     char *ws = nullptr;
     size_t ws_bytes = 0;
     unsigned long long *h_small = nullptr;  // 4 words
+    int *d_guard = nullptr;                 // 4 ints: CSR validation, distance-range check
+    // Live objects made on this context.  np_destroy releases what they hold on the device
+    // and detaches them: their handles stay valid for np_graph_free / np_batch_free, which
+    // then only delete the host object, and every other call rejects them.
+    std::vector<np_graph *> graphs;
+    std::vector<np_batch *> batches;
+    std::vector<void *> allocs;             // np_dev_alloc
 };
+
+template <class T>
+static void unlink_from(std::vector<T *> &v, T *p) {
+    auto it = std::find(v.begin(), v.end(), p);
+    if (it != v.end()) v.erase(it);
+}
 
 struct np_graph {
     np_ctx *ctx = nullptr;
@@ -153,6 +172,7 @@ struct np_graph {
     Space sp;  // single-source solver state (lazily allocated)
     bool sp_ready = false;
     int *out = nullptr;  // un-permuted distances (renumbered graphs)
+    int max_w = 0;       // largest edge weight (validated: 0 <= w < INF)
 
     // solver id of original vertex v (one 4-byte read when renumbered)
     int solver_id(int v, hipStream_t s) const {
@@ -253,6 +273,8 @@ static DevState make_state(np_ctx *c, Space &s, const int *row, const int *col, 
     st.inf = inf;
     st.ctrl = c->ctrl;
     st.kshift = -1;
+    st.sbuf = s.sbuf;
+    st.sbcnt = s.sbcnt;
     return st;
 }
 
@@ -267,6 +289,9 @@ static int validate_sched(const np_sched *in, np_sched *out) {
     if (out->frontier < 0 || out->frontier > 1) { set_error("bad frontier mode"); return NP_ERR_ARG; }
     if (out->wgs_per_cu < 0 || out->graph_chunk < 0 || out->max_iters < 0 || out->delta < 0) {
         set_error("negative schedule field"); return NP_ERR_ARG;
+    }
+    if (out->scheme < NP_SCHEME_CONSOLIDATE || out->scheme > NP_SCHEME_BFS_HIER) {
+        set_error("scheme must be 0..8"); return NP_ERR_ARG;
     }
     return NP_OK;
 }
@@ -288,6 +313,11 @@ static SchedDev make_schedd(np_ctx *c, const np_sched &s, int n_feat) {
     d.n_feat = n_feat;
     d.mlp = (d.learned && c->has_model) ? c->model : nullptr;
     if (d.learned && !c->has_model) d.learned = 0;  // no network: the heuristic runs
+    d.scheme = s.scheme;
+    if (d.scheme != NP_SCHEME_CONSOLIDATE) {
+        // the reference's other schedules: fixed heavy threshold, scan frontier, no window
+        d.learned = 0; d.mlp = nullptr; d.level = 1; d.push = 0; d.delta = 0;
+    }
     return d;
 }
 
@@ -301,6 +331,37 @@ static int persistent_wgs(np_ctx *c, const np_sched &s, bool batch, bool unit, b
 static int read_ctrl(np_ctx *c) {
     HIPCHK(hipMemcpyAsync(c->h_ctrl, c->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, c->stream));
     return ctx_sync(c);
+}
+
+// Whether a solve over graphs of up to n vertices and largest weight max_w needs the
+// distance-range check.
+static void set_guard(Space &s, long long n, int max_w) {
+    s.max_w = std::max(s.max_w, max_w);
+    s.guard = s.guard || (n - 1) * (long long)max_w >= (long long)INF;
+}
+
+// The distance-range check after a solve of SSSP distances (not BFS levels, which stay
+// below n).  The reference's relaxation `if (cost[v] > alt) atomicMin` (sssp_kernel.cu:46-52)
+// never stores alt >= INF = 2^30, so a vertex whose distance would reach INF is reported as
+// unreached.  When (vertices - 1) x (largest weight) >= INF this can happen: the largest
+// finite distance is read back, and only when it is within one weight of INF are the edges
+// swept for a reached tail whose target stayed at INF (NP_ERR_CAPACITY).  Weights are below
+// INF (checked at graph creation), so dist[u] + w never overflows int32.
+static int distance_guard(np_ctx *c, const Space &s, const DevState &st, bool batch) {
+    if (!s.guard) return NP_OK;
+    int *h = (int *)c->h_small;
+    HIPCHK(launch_max_finite(st, c->d_guard, c->stream));
+    HIPCHK(hipMemcpyAsync(h, c->d_guard, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if ((long long)h[0] + s.max_w < (long long)INF) return NP_OK;
+    HIPCHK(launch_dist_escape(st, batch, c->d_guard, c->stream));
+    HIPCHK(hipMemcpyAsync(h, c->d_guard, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (h[0] > 0) {
+        set_error("a shortest distance reaches INF = 2^30 (%d relaxations out of the int32 range of the reference)", h[0]);
+        return NP_ERR_CAPACITY;
+    }
+    return NP_OK;
 }
 
 // Captures (or reuses) the graph drive's chunk of sch.graph_chunk iterations for this state.
@@ -345,6 +406,8 @@ static int run_drive(np_ctx *c, Space &s, const DevState &st, const np_sched &sc
     int rc = NP_OK;
     float ms = 0.f;
     int drive = sch.drive;
+    // the reference's other schedules are per-iteration launches
+    if (sd.scheme != NP_SCHEME_CONSOLIDATE && drive == NP_DRIVE_PERSISTENT) drive = NP_DRIVE_HOSTLOOP;
     if (drive == NP_DRIVE_PERSISTENT) {
         int wgs = persistent_wgs(c, sch, batch, unit, st.kshift >= 0);
         HIPCHK(hipEventRecord(c->ev0, c->stream));
@@ -392,6 +455,7 @@ static int run_drive(np_ctx *c, Space &s, const DevState &st, const np_sched &sc
     rc = h.error ? h.error : NP_OK;
     if (rc == NP_ERR_TIMEOUT) set_error("a grid barrier timed out; the kernel exited early");
     if (rc == NP_ERR_NOT_CONVERGED) set_error("iteration cap reached");
+    if (rc == NP_OK && !unit) rc = distance_guard(c, s, st, batch);
     if (stats) {
         std::memset(stats, 0, sizeof(*stats));
         stats->iterations = h.iters;
@@ -411,6 +475,7 @@ static int graph_space(np_ctx *c, np_graph *g) {
     std::memcpy(s.hist, g->hist, sizeof(s.hist));
     s.cls = g->cls;
     s.own_cls = false;
+    set_guard(s, g->n, g->max_w);
     int rc = space_alloc(c, s, true);
     if (rc) { s.release(); return rc; }
     if ((rc = ensure_src(c, s, 1))) return rc;
@@ -418,17 +483,57 @@ static int graph_space(np_ctx *c, np_graph *g) {
     return NP_OK;
 }
 
+// BFS_REC_GPU solution 0 (bfs_flat_gpu, bfs_rec_wrapper.cu:111-133): one flat pass per level
+// with a one-byte "changed" readback, until a level writes nothing.  The level array holds
+// UNDEFINED and 0 at the source (launch_init_solve).
+static int run_bfs_flat(np_ctx *c, const DevState &st, int n, np_stats *stats) {
+    const int wgs = c->num_cus * 8;
+    int *changed = &c->d_guard[3];
+    int *h = (int *)c->h_small;
+    int level = 0;
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    for (;; ++level) {
+        if (level >= n) { set_error("BFS deeper than n levels"); return NP_ERR_NOT_CONVERGED; }
+        HIPCHK(launch_fill_i32(changed, 1, 0, c->stream));
+        HIPCHK(launch_bfs_flat(st, level, changed, wgs, c->stream));
+        HIPCHK(hipMemcpyAsync(h, changed, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (!h[0]) break;
+    }
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    HIPCHK(hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_iters = level + 1;
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        stats->iterations = level + 1;
+        stats->kernel_ms = ms;
+        stats->drive = NP_DRIVE_HOSTLOOP;
+    }
+    return NP_OK;
+}
+
 static int solve_single(np_ctx *c, np_graph *g, int src, const np_sched *sched, int *d_dist_user,
                         int *h_dist, bool unit, np_stats *stats) {
     if (!c || !g) { set_error("null context or graph"); return NP_ERR_ARG; }
-    if (g->ctx != c) { set_error("graph belongs to another context"); return NP_ERR_ARG; }
+    if (g->ctx != c) { set_error("graph belongs to another context or its context was destroyed"); return NP_ERR_ARG; }
     HIPCHK(hipSetDevice(c->device));
     if (src < 0 || src >= g->n) { set_error("source %d out of range [0,%d)", src, g->n); return NP_ERR_ARG; }
     np_sched sch;
     int rc = validate_sched(sched, &sch);
     if (rc) return rc;
+    if (!unit && (sch.scheme == NP_SCHEME_BFS_FLAT || sch.scheme == NP_SCHEME_BFS_HIER)) {
+        set_error("scheme %d computes BFS levels (np_bfs) only", sch.scheme);
+        return NP_ERR_ARG;
+    }
     if ((rc = graph_space(c, g))) return rc;
     Space &s = g->sp;
+    if (sch.scheme == NP_SCHEME_GLOBAL_BUFFER && !s.sbuf) {
+        // at most every queued vertex is heavy in one expansion: the queue's capacity
+        HIPCHK(dalloc(&s.sbuf, (size_t)std::max(1ll, s.qcap)));
+        HIPCHK(dalloc(&s.sbcnt, 3));
+    }
     const int ssrc = g->solver_id(src, c->stream);
     if (ssrc < 0) { set_error("source mapping failed"); return NP_ERR_HIP; }
     // a renumbered graph is solved in its own buffer, then mapped back to original ids
@@ -438,7 +543,7 @@ static int solve_single(np_ctx *c, np_graph *g, int src, const np_sched *sched, 
     SchedDev sd = make_schedd(c, sch, g->n);
     HIPCHK(hipMemcpyAsync(s.src_slots, &ssrc, sizeof(int), hipMemcpyHostToDevice, c->stream));
     HIPCHK(launch_init_solve(st, s.src_slots, 1, sd.push, sd.delta, c->stream));
-    rc = run_drive(c, s, st, sch, sd, false, unit, stats);
+    rc = sch.scheme == NP_SCHEME_BFS_FLAT ? run_bfs_flat(c, st, g->n, stats) : run_drive(c, s, st, sch, sd, false, unit, stats);
     if (rc) return rc;
     const int *result = dist;
     if (g->rl.active) {
@@ -456,20 +561,34 @@ static int solve_single(np_ctx *c, np_graph *g, int src, const np_sched *sched, 
     return ctx_sync(c);
 }
 
+// Validates a device CSR (structure, and weights: none negative, every one below INF so
+// that dist[u] + w stays in int32), renumbers it when skewed, classes its vertices by
+// out-degree and registers it with the context.
 static int finish_graph(np_ctx *c, np_graph *g) {
     int *hist = nullptr;
-    int bad = 0;
-    HIPCHK(dalloc(&hist, NCLS + 1));
-    hipError_t e = launch_fill_i32(hist, NCLS + 1, 0, c->stream);
-    if (e == hipSuccess) e = launch_validate_csr(g->row, g->col, g->n, g->m, hist + NCLS, c->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(&bad, hist + NCLS, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+    int chk[3] = {0, 0, 0};
+    HIPCHK(dalloc(&hist, NCLS + 3));
+    hipError_t e = launch_fill_i32(hist, NCLS + 3, 0, c->stream);
+    if (e == hipSuccess) e = launch_validate_csr(g->row, g->col, g->w, g->n, g->m, hist + NCLS, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(chk, hist + NCLS, sizeof(chk), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) { dfree(hist); return hip_fail(e, "CSR validation", __LINE__); }
-    if (bad) {
+    if (chk[0]) {
         dfree(hist);
-        set_error("invalid CSR: %d violations of row[0]=0, row non-decreasing, row[n]=m, 0<=col<n", bad);
+        set_error("invalid CSR: %d violations of row[0]=0, row non-decreasing, row[n]=m, 0<=col<n", chk[0]);
         return NP_ERR_ARG;
     }
+    if (chk[1]) {
+        dfree(hist);
+        set_error("%d negative edge weights: shortest paths need weights >= 0", chk[1]);
+        return NP_ERR_ARG;
+    }
+    if (chk[2] >= INF) {
+        dfree(hist);
+        set_error("edge weight %d >= INF = 2^30: dist[u] + w could overflow the int32 distances", chk[2]);
+        return NP_ERR_CAPACITY;
+    }
+    g->max_w = chk[2];
     g->srow = g->row; g->scol = g->col; g->sw = g->w;
     if (c->relabel != 0) {
         const double min_skew = c->relabel == 1 ? -1.0 : 0.4;
@@ -484,8 +603,10 @@ static int finish_graph(np_ctx *c, np_graph *g) {
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     dfree(hist);
     if (e != hipSuccess) return hip_fail(e, "degree classes", __LINE__);
+    c->graphs.push_back(g);
     return NP_OK;
 }
+
 
 static void init_model_params(float *p) {
     // He-style deterministic initialisation from SplitMix64 draws.
@@ -502,6 +623,26 @@ static void init_model_params(float *p) {
         }
         for (int i = 0; i < bias[l]; ++i) p[off++] = 0.f;
     }
+}
+
+// Device resources of a graph / batch (the host object stays).
+static void graph_release(np_graph *g) {
+    g->sp.release();
+    g->sp_ready = false;
+    if (g->owns) { dfree(g->row); dfree(g->col); dfree(g->w); }
+    dfree(g->rl.row); dfree(g->rl.col); dfree(g->rl.w); dfree(g->rl.old2new);
+    g->rl.active = false;
+    dfree(g->cls);
+    dfree(g->out);
+}
+
+static void batch_release(np_batch *b) {
+    b->sp.release();
+    for (Pack *pk : b->packs) { pk->sp.release(); delete pk; }
+    b->packs.clear();
+    if (b->owns_graph) { dfree(b->row); dfree(b->col); dfree(b->w); }
+    dfree(b->out);
+    b->nconcat = 0;
 }
 
 // ------------------------------------------------------------------------------------
@@ -542,6 +683,7 @@ int np_create(int device, np_ctx **out) {
     if (e == hipSuccess) e = dalloc(&c->trace, TRACE_CAP);
     if (e == hipSuccess) e = dalloc(&c->model, NP_MLP_NPARAM);
     if (e == hipSuccess) e = dalloc(&c->d_work2, 4);
+    if (e == hipSuccess) e = dalloc(&c->d_guard, 4);
     if (e != hipSuccess) {
         int rc = hip_fail(e, "context setup", __LINE__);
         np_destroy(c);
@@ -590,8 +732,14 @@ int np_destroy(np_ctx *c) {
     if (!c) return NP_OK;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    // batches first: they read their graphs
+    for (np_batch *b : c->batches) { batch_release(b); b->ctx = nullptr; }
+    for (np_graph *g : c->graphs) { graph_release(g); g->ctx = nullptr; }
+    for (void *p : c->allocs) (void)hipFree(p);
+    c->batches.clear(); c->graphs.clear(); c->allocs.clear();
+    c->step_g = nullptr;
     dfree(c->ctrl); dfree(c->trace); dfree(c->model); dfree(c->adam_m); dfree(c->adam_v);
-    dfree(c->d_work2); dfree(c->snap_dist); dfree(c->snap_bm);
+    dfree(c->d_work2); dfree(c->d_guard); dfree(c->snap_dist); dfree(c->snap_bm);
     if (c->h_ctrl) (void)hipHostFree(c->h_ctrl);
     if (c->h_small) (void)hipHostFree(c->h_small);
     if (c->ws) (void)hipFree(c->ws);
@@ -615,12 +763,17 @@ int np_dev_alloc(np_ctx *c, uint64_t bytes, void **d_ptr) {
     HIPCHK(hipSetDevice(c->device));
     hipError_t e = hipMalloc(d_ptr, bytes ? bytes : 1);
     if (e != hipSuccess) { set_error("hipMalloc(%llu): %s", (unsigned long long)bytes, hipGetErrorString(e)); return NP_ERR_NOMEM; }
+    c->allocs.push_back(*d_ptr);
     return NP_OK;
 }
 
 int np_dev_free(np_ctx *c, void *d_ptr) {
     if (!c) { set_error("null context"); return NP_ERR_ARG; }
     if (d_ptr) {
+        auto it = std::find(c->allocs.begin(), c->allocs.end(), d_ptr);
+        if (it == c->allocs.end()) { set_error("pointer was not allocated by np_dev_alloc on this context"); return NP_ERR_ARG; }
+        c->allocs.erase(it);
+        HIPCHK(hipSetDevice(c->device));
         HIPCHK(hipStreamSynchronize(c->stream));
         HIPCHK(hipFree(d_ptr));
     }
@@ -650,12 +803,35 @@ static int new_graph(np_ctx *c, int32_t n, int64_t m, np_graph **out) {
     return NP_OK;
 }
 
+int np_csr_validate(int32_t n, int64_t m, const int32_t *row, const int32_t *col, const int32_t *w,
+                    int32_t *max_w) {
+    if (n < 0 || m < 0 || !row || (m > 0 && !col)) { set_error("bad CSR arguments"); return NP_ERR_ARG; }
+    if (m > 0x7FFFFFFFll) { set_error("m = %lld exceeds the int32 CSR of the reference", (long long)m); return NP_ERR_CAPACITY; }
+    if (row[0] != 0 || row[n] != m) { set_error("row[0] must be 0 and row[n] must be m"); return NP_ERR_ARG; }
+    for (int32_t v = 0; v < n; ++v)
+        if (row[v + 1] < row[v]) { set_error("row decreases at vertex %d", v); return NP_ERR_ARG; }
+    int32_t mx = 0;
+    for (int64_t e = 0; e < m; ++e) {
+        if (col[e] < 0 || col[e] >= n) { set_error("col[%lld] = %d outside [0,%d)", (long long)e, col[e], n); return NP_ERR_ARG; }
+        if (w) {
+            if (w[e] < 0) { set_error("negative weight %d at edge %lld: shortest paths need weights >= 0", w[e], (long long)e); return NP_ERR_ARG; }
+            mx = std::max(mx, w[e]);
+        }
+    }
+    if (mx >= INF) {
+        set_error("edge weight %d >= INF = 2^30: dist[u] + w could overflow the int32 distances", mx);
+        return NP_ERR_CAPACITY;
+    }
+    if (max_w) *max_w = mx;
+    return NP_OK;
+}
+
 int np_graph_upload(np_ctx *c, int32_t n, int64_t m, const int32_t *row, const int32_t *col,
                     const int32_t *w, np_graph **out) {
     int rc = new_graph(c, n, m, out);
     if (rc) return rc;
     if (!row || (m > 0 && (!col || !w))) { set_error("null CSR array"); return NP_ERR_ARG; }
-    if (row[0] != 0 || row[n] != m) { set_error("row[0] must be 0 and row[n] must be m"); return NP_ERR_ARG; }
+    if ((rc = np_csr_validate(n, m, row, col, w, nullptr))) return rc;
     HIPCHK(hipSetDevice(c->device));
     np_graph *g = new np_graph();
     g->ctx = c; g->n = n; g->m = m; g->owns = true;
@@ -694,6 +870,7 @@ int np_graph_info(const np_graph *g, int32_t *n, int64_t *m) {
 
 int np_graph_download(np_ctx *c, const np_graph *g, int32_t *row, int32_t *col, int32_t *w) {
     if (!c || !g) { set_error("null argument"); return NP_ERR_ARG; }
+    if (g->ctx != c) { set_error("graph belongs to another context or its context was destroyed"); return NP_ERR_ARG; }
     HIPCHK(hipSetDevice(c->device));
     if (row) HIPCHK(hipMemcpyAsync(row, g->row, sizeof(int) * ((size_t)g->n + 1), hipMemcpyDeviceToHost, c->stream));
     if (col && g->m) HIPCHK(hipMemcpyAsync(col, g->col, sizeof(int) * (size_t)g->m, hipMemcpyDeviceToHost, c->stream));
@@ -711,13 +888,12 @@ int np_graph_device_ptrs(const np_graph *g, const int32_t **row, const int32_t *
 
 int np_graph_free(np_graph *g) {
     if (!g) return NP_OK;
-    if (g->ctx) (void)hipSetDevice(g->ctx->device);
-    if (g->ctx && g->ctx->step_g == g) g->ctx->step_g = nullptr;
-    g->sp.release();
-    if (g->owns) { dfree(g->row); dfree(g->col); dfree(g->w); }
-    dfree(g->rl.row); dfree(g->rl.col); dfree(g->rl.w); dfree(g->rl.old2new);
-    dfree(g->cls);
-    dfree(g->out);
+    if (np_ctx *c = g->ctx) {  // a graph whose context was destroyed holds nothing on the device
+        (void)hipSetDevice(c->device);
+        if (c->step_g == g) c->step_g = nullptr;
+        unlink_from(c->graphs, g);
+        graph_release(g);
+    }
     delete g;
     return NP_OK;
 }
@@ -739,6 +915,7 @@ int np_bfs(np_ctx *c, np_graph *g, int32_t src, const np_sched *sched, int32_t *
 
 int np_reached_work(np_ctx *c, np_graph *g, const int32_t *d_dist, int32_t inf, int64_t *n_r, int64_t *m_r) {
     if (!c || !g || !d_dist) { set_error("null argument"); return NP_ERR_ARG; }
+    if (g->ctx != c) { set_error("graph belongs to another context or its context was destroyed"); return NP_ERR_ARG; }
     HIPCHK(hipSetDevice(c->device));
     unsigned long long h[2] = {0, 0};
     HIPCHK(launch_reached_work(g->row, d_dist, g->n, inf, c->d_work2, c->stream));
@@ -1233,6 +1410,7 @@ static int build_pack(np_ctx *c, np_batch *b, np_graph *g, const std::vector<int
     std::memcpy(s.hist, g->hist, sizeof(s.hist));
     s.cls = g->cls;
     s.own_cls = false;
+    set_guard(s, g->n, g->max_w);
     int rc = space_alloc(c, s, false);
     if (rc) return rc;
     HIPCHK(dalloc(&s.dist, (size_t)g->n << ks));
@@ -1297,6 +1475,7 @@ static int build_concat(np_ctx *c, np_batch *b, const std::vector<int> &ids, np_
         b->base[ids[t]] = (int)acc;
         inst[t] = make_int2((int)acc, (int)VB[gi[t]]);
         for (int k = 0; k < NCLS; ++k) s.hist[k] += g->hist[k];
+        set_guard(s, g->n, g->max_w);
         acc += ((long long)g->n + G - 1) / G * G;
         if (acc > 0x7FFFFFFFll - G) { set_error("batch exceeds 2^31 vertex slots"); return NP_ERR_CAPACITY; }
     }
@@ -1370,6 +1549,7 @@ int np_batch_create(np_ctx *c, int32_t count, np_graph *const *graphs, const int
         rc = build_concat(c, b, rest, graphs, srcs);
     }
     if (rc) { np_batch_free(b); return rc; }
+    c->batches.push_back(b);
     *out = b;
     return NP_OK;
 }
@@ -1438,6 +1618,8 @@ static int run_parts_concurrent(np_ctx *c, std::vector<PartJob> &jobs, const np_
     }
     if (rc == NP_ERR_NOT_CONVERGED) set_error("iteration cap reached");
     if (rc == NP_ERR_TIMEOUT) set_error("a grid barrier timed out; the kernel exited early");
+    for (PartJob &j : jobs)
+        if (rc == NP_OK && !unit) rc = distance_guard(c, *j.s, j.st, j.batch);
     return rc;
 }
 
@@ -1447,6 +1629,10 @@ int np_batch_solve(np_ctx *c, np_batch *b, const np_sched *sched, np_stats *stat
     np_sched sch;
     int rc = validate_sched(sched, &sch);
     if (rc) return rc;
+    if (sch.scheme != NP_SCHEME_CONSOLIDATE) {
+        set_error("batches run the consolidation bins (scheme 0) only");
+        return NP_ERR_ARG;
+    }
     const bool unit = b->unit != 0;
     const int inf = unit ? UNDEFINED : INF;
     np_stats acc;
@@ -1549,6 +1735,7 @@ static int batch_result(np_ctx *c, np_batch *b, int i, const int **res) {
 
 int np_batch_get(np_ctx *c, np_batch *b, int32_t i, int32_t *dist_out) {
     if (!c || !b || !dist_out || i < 0 || i >= b->count) { set_error("bad argument"); return NP_ERR_ARG; }
+    if (b->ctx != c) { set_error("batch belongs to another context or its context was destroyed"); return NP_ERR_ARG; }
     HIPCHK(hipSetDevice(c->device));
     const int *d = nullptr;
     int rc = batch_result(c, b, i, &d);
@@ -1559,6 +1746,7 @@ int np_batch_get(np_ctx *c, np_batch *b, int32_t i, int32_t *dist_out) {
 
 int np_batch_device_dist(np_batch *b, int32_t i, const int32_t **d_dist) {
     if (!b || !d_dist || i < 0 || i >= b->count) { set_error("bad argument"); return NP_ERR_ARG; }
+    if (!b->ctx) { set_error("the batch's context was destroyed"); return NP_ERR_ARG; }
     int rc = batch_result(b->ctx, b, i, d_dist);
     if (rc) return rc;
     return ctx_sync(b->ctx);
@@ -1573,12 +1761,11 @@ int np_batch_layout(const np_batch *b, int32_t *packs, int32_t *concatenated) {
 
 int np_batch_free(np_batch *b) {
     if (!b) return NP_OK;
-    if (b->ctx) (void)hipSetDevice(b->ctx->device);
-    b->sp.release();
-    for (Pack *pk : b->packs) { pk->sp.release(); delete pk; }
-    b->packs.clear();
-    if (b->owns_graph) { dfree(b->row); dfree(b->col); dfree(b->w); }
-    dfree(b->out);
+    if (np_ctx *c = b->ctx) {  // a batch whose context was destroyed holds nothing on the device
+        (void)hipSetDevice(c->device);
+        unlink_from(c->batches, b);
+        batch_release(b);
+    }
     delete b;
     return NP_OK;
 }
@@ -1885,11 +2072,12 @@ static int tune_window(const std::function<int(const np_sched &, double *)> &eva
 
 int np_autotune(np_ctx *c, np_graph *g, const int32_t *srcs, int32_t nsrc, const np_sched *base,
                 np_sched *best, double *best_ms) {
-    if (!c || !g || !srcs || nsrc < 1 || !best) { set_error("bad arguments"); return NP_ERR_ARG; }
+    if (!c || !g || g->ctx != c || !srcs || nsrc < 1 || !best) { set_error("bad arguments"); return NP_ERR_ARG; }
     HIPCHK(hipSetDevice(c->device));
     np_sched b0;
     int rc = validate_sched(base, &b0);
     if (rc) return rc;
+    b0.scheme = NP_SCHEME_CONSOLIDATE;  // the search is over the consolidation bins
     int occ = persistent_max_wgs_per_cu(false, false, b0.frontier == NP_FRONTIER_PUSH, false);
     std::vector<int> wgs_opts = {0};
     if (occ > 2) wgs_opts.push_back(2);
@@ -1951,6 +2139,7 @@ int np_batch_autotune(np_ctx *c, np_batch *b, const np_sched *base, np_sched *be
     np_sched b0;
     int rc = validate_sched(base, &b0);
     if (rc) return rc;
+    b0.scheme = NP_SCHEME_CONSOLIDATE;
     int occ = persistent_max_wgs_per_cu(true, b->unit != 0, false, !b->packs.empty());
     std::vector<int> wgs_opts = {0};
     if (occ > 2) wgs_opts.push_back(2);

@@ -151,6 +151,7 @@ static int read_graph() {
 static void validation_bfs(int *level) {
     const unsigned UNDEF = 0x7FFFFFFF;
     for (int i = 0; i < noNodeTotal; ++i) level[i] = (int)UNDEF;
+    if (source < 0 || source >= noNodeTotal) return;  // empty graph: nothing reached
     level[source] = 0;
     std::queue<int> q;
     q.push(source);
@@ -265,7 +266,7 @@ int main(int argc, char *argv[]) {
     printf("\n===MAIN=== :: [num_nodes,num_edges] = %u, %u\n", noNodeTotal, noEdgeTotal);
     for (int i = 0; i < N; ++i) {
         for (int k = 0; k < noNodeTotal; ++k) graph.levelArray[k] = 0x7FFFFFFF;
-        graph.levelArray[source] = 0;
+        if (source >= 0 && source < noNodeTotal) graph.levelArray[source] = 0;
         BFS_REC_GPU();
     }
     int *cpu = new int[noNodeTotal > 0 ? noNodeTotal : 1];

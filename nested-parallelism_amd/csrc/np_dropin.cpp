@@ -93,20 +93,47 @@ int env_level(int dflt) {
     return dflt;
 }
 
-// Schedules for config.solution (apps/sssp/src/gpu/sssp.cpp:31-38).  Every solution
-// relaxes with the same arithmetic; they differ in how frontier vertices are mapped to
-// workers, which here is the bin configuration:
-//   0 thread queue, 1 dual queue, 2 shared delayed buffer, 3 global delayed buffer,
-//   4 per-vertex nested launch, 5 per-block nested launch, 6 consolidation.
+// Schedules for config.solution (apps/sssp/src/gpu/sssp.cpp:31-38, sssp_wrapper.cu:553-570).
+// Every solution relaxes with the same arithmetic; they differ in how frontier vertices are
+// mapped to workers.  Solutions 0-5 keep the reference's shape -- a frontier queue per
+// iteration, a lane per light vertex, heavy vertices (out-degree >= THRESHOLD 64) handed on --
+// and run on the host loop with one readback per iteration, as sssp_wrapper.cu does:
+//   0 thread queue            NP_SCHEME_THREAD        (sssp_tqueue_gpu, :151-191)
+//   1 dual queue              NP_SCHEME_DUAL_QUEUE    (sssp_dual_queue_gpu, :193-293)
+//   2 shared delayed buffer   NP_SCHEME_LDS_BUFFER    (:295-331)
+//   3 global delayed buffer   NP_SCHEME_GLOBAL_BUFFER (:333-396)
+//   4 nested launch per heavy vertex   NP_SCHEME_WAVE_NESTED (sssp_np_naive_gpu, :398-433)
+//   5 nested launch per block          NP_SCHEME_WG_NESTED   (sssp_np_opt_gpu, :435-473)
+//   6 consolidation           NP_SCHEME_CONSOLIDATE at CONSOLIDATE_LEVEL (:475-546)
 bool sssp_schedule(int solution, np_sched *s) {
+    static const int scheme[6] = {NP_SCHEME_THREAD, NP_SCHEME_DUAL_QUEUE, NP_SCHEME_LDS_BUFFER,
+                                  NP_SCHEME_GLOBAL_BUFFER, NP_SCHEME_WAVE_NESTED, NP_SCHEME_WG_NESTED};
+    if (solution == 6) { np_sched_default(env_level(CONSOLIDATE_LEVEL), s); return true; }
+    if (solution < 0 || solution > 5) return false;
+    np_sched_default(1, s);
+    s->scheme = scheme[solution];
+    s->drive = NP_DRIVE_HOSTLOOP;
+    return true;
+}
+
+// Schedules for BFS_REC_GPU's config.solution (apps/bfs-rec/src/gpu/bfs_rec.cpp:31-34,
+// bfs_rec_wrapper.cu:241-262):
+//   0 flat                    NP_SCHEME_BFS_FLAT: a scan of every vertex per level
+//   1 naive recursion         a workgroup per frontier vertex (the reference's one child
+//                             block per vertex), improved children appended directly
+//                             (the push frontier: no scan, as a nested launch needs none)
+//   2 hierarchical recursion  NP_SCHEME_BFS_HIER: two levels per expansion
+//   3 consolidation           NP_SCHEME_CONSOLIDATE at CONSOLIDATE_LEVEL
+bool bfs_schedule(int solution, np_sched *s) {
     switch (solution) {
-        case 0: np_sched_default(0, s); s->log2_t_lo = 30; s->log2_t_hi = 30; return true;
-        case 1: np_sched_default(1, s); return true;
-        case 2: np_sched_default(1, s); return true;
-        case 3: np_sched_default(2, s); return true;
-        case 4: np_sched_default(0, s); return true;
-        case 5: np_sched_default(1, s); return true;
-        case 6: np_sched_default(env_level(CONSOLIDATE_LEVEL), s); return true;
+        case 0: np_sched_default(1, s); s->scheme = NP_SCHEME_BFS_FLAT; s->drive = NP_DRIVE_HOSTLOOP; return true;
+        case 1:
+            np_sched_default(1, s);
+            s->log2_t_lo = 0; s->log2_t_hi = 0;   // every vertex of out-degree >= 1 in the workgroup bin
+            s->frontier = NP_FRONTIER_PUSH; s->drive = NP_DRIVE_HOSTLOOP;
+            return true;
+        case 2: np_sched_default(1, s); s->scheme = NP_SCHEME_BFS_HIER; s->drive = NP_DRIVE_HOSTLOOP; return true;
+        case 3: np_sched_default(env_level(BFS_CONSOLIDATE_LEVEL), s); return true;
         default: return false;
     }
 }
@@ -233,6 +260,11 @@ void SSSP_GPU() {
     end_time = gettime();
     init_time += end_time - start_time;
     if (DEBUG) fprintf(stderr, "Choose CUDA device: %d\n", config.device_num);
+    if (noNodeTotal <= 0) return;  // an empty graph: nothing to solve
+    if (source < 0 || source >= noNodeTotal) {
+        fprintf(stderr, "Error: source %d outside the graph's %d vertices\n", source + 1, noNodeTotal);
+        exit(1);
+    }
     // host-side initialisation of prepare_gpu (sssp_wrapper.cu:92-96)
     for (int i = 0; i < noNodeTotal; i++) graph.costArray[i] = NP_INF;
     if (graph.update) graph.update[source] = 1;
@@ -255,6 +287,7 @@ void SSSP_GPU() {
 }
 
 void BFS_REC_GPU() {
+    if (noNodeTotal <= 0) return;  // an empty graph: nothing to solve
     start_time = gettime_ms();
     np_ctx *c = ctx_for_device();
     end_time = gettime_ms();
@@ -269,11 +302,8 @@ void BFS_REC_GPU() {
     np_stats st;
     memset(&st, 0, sizeof st);
     double t0 = gettime_ms();
-    if (config.solution >= 0 && config.solution <= 3) {
-        // solutions 0 flat, 1 naive, 2 hierarchical, 3 consolidated
-        // (apps/bfs-rec/src/gpu/bfs_rec.cpp:31-34): one unit-weight solve each
-        np_sched s;
-        np_sched_default(config.solution == 3 ? env_level(BFS_CONSOLIDATE_LEVEL) : 1, &s);
+    np_sched s;
+    if (bfs_schedule(config.solution, &s)) {
         if (np_bfs(c, g, source, &s, graph.levelArray, &st) != NP_OK) fail("BFS");
         if (DEBUG) printf("===> GPU #%d - BFS levels in %d iterations.\n", config.solution + 1, st.iterations);
     }

@@ -53,6 +53,10 @@ struct DevState {
     int kshift;               // -1: one distance per slot
     unsigned *mflag;
     unsigned *qmask[2];
+    // NP_SCHEME_GLOBAL_BUFFER: heavy vertices of one expansion and the buffer's fill
+    // counters [3], rotating by iteration
+    int *sbuf;
+    int *sbcnt;
 };
 
 // Largest number of sources packed into one distance row.
@@ -68,6 +72,7 @@ struct SchedDev {
     int n_feat;             // vertex count used by the features
     const float *mlp;       // NP_MLP_NPARAM device floats
     np_trace_rec *trace;
+    int scheme;             // NP_SCHEME_* (0: the consolidation bins)
 };
 
 // ---- kernels.hip: solve drivers ----------------------------------------------------
@@ -75,8 +80,15 @@ struct SchedDev {
 hipError_t launch_fill_i32(int *p, long long n, int v, hipStream_t s);
 hipError_t launch_fill_u32(unsigned *p, long long n, unsigned v, hipStream_t s);
 hipError_t launch_degree_class(const int *row, int n, unsigned char *cls, int *hist, hipStream_t s);
-hipError_t launch_validate_csr(const int *row, const int *col, int n, long long m, int *bad,
+// out3[0] = CSR violations, out3[1] = negative weights, out3[2] = largest weight (w may be
+// null: no weight checks); out3 must hold zeros.
+hipError_t launch_validate_csr(const int *row, const int *col, const int *w, int n, long long m, int *out3,
                                hipStream_t s);
+// *out = largest finite distance of the state's buffer (0 when none).
+hipError_t launch_max_finite(const DevState &st, int *out, hipStream_t s);
+// *bad = reached edges u -> v with dist[v] = inf and dist[u] + w >= inf (a distance that the
+// int32 range below inf cannot hold); batch: concatenated slots.
+hipError_t launch_dist_escape(const DevState &st, bool batch, int *bad, hipStream_t s);
 hipError_t launch_init_solve(const DevState &st, const int *d_src_slots, int nsrc, int push, int delta,
                              hipStream_t s);
 hipError_t launch_reached_work(const int *row, const int *dist, int n, int inf,
@@ -105,6 +117,9 @@ hipError_t launch_decide(const DevState &st, const SchedDev &sp, int it_off, hip
 hipError_t launch_expand(const DevState &st, const SchedDev &sp, int it_off, bool batch, bool unit,
                          int wgs, hipStream_t s);
 hipError_t launch_advance(const DevState &st, int k, hipStream_t s);
+// NP_SCHEME_BFS_FLAT: one level-synchronous pass over every vertex (levels in st.dist);
+// *changed (device) is set when a level was written.
+hipError_t launch_bfs_flat(const DevState &st, int level, int *changed, int wgs, hipStream_t s);
 // Stepping: force the next expansion's schedule (writes ctrl->cfg_*).
 hipError_t launch_set_cfg(const DevState &st, int level, int klo, int khi, hipStream_t s);
 

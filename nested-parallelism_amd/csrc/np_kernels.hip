@@ -697,6 +697,261 @@ __global__ void k_set_cfg(DevState st, int level, int klo, int khi) {
 }
 
 // ------------------------------------------------------------------------------------
+// the reference's other schedules (SSSP_GPU solutions 0-5, BFS_REC_GPU solutions 0 and 2)
+// ------------------------------------------------------------------------------------
+// They run on the host-loop and graph drives after the scan compaction and k_decide of the
+// consolidated path, over the same class-segmented queue; they differ from it, and from one
+// another, only in how queued vertices are mapped to workers (sssp_wrapper.cu:151-473,
+// sssp_kernel.cu:34-327; bfs_rec_kernel.cu:41-140).  A vertex is heavy when its out-degree is
+// at least 2^cfg_klo (THRESHOLD 64 = 2^6, sssp_kernel.cu:7), i.e. its class is >= cfg_klo + 1.
+// Every relaxation is the reference's: alt = dist[u] + w; if (dist[v] > alt) atomicMin.
+
+// One lane walks all edges of one vertex (the thread loop of unorder_threadQueue_kernel,
+// sssp_kernel.cu:34-55), UNR edges' loads in flight at a time.
+template <bool UNIT>
+__device__ __forceinline__ void relax_serial(const DevState &st, const PushCtx &pc, int slot) {
+    const int e0 = st.row[slot], e1 = st.row[slot + 1];
+    const int du = ld_coherent(st.dist + slot);
+    for (int base = e0; base < e1; base += UNR) {
+        int tv[UNR], alt[UNR], cur[UNR];
+#pragma unroll
+        for (int k = 0; k < UNR; ++k) {
+            const int e = base + k < e1 ? base + k : e0;
+            tv[k] = st.col[e];
+            alt[k] = du + (UNIT ? 1 : st.w[e]);
+        }
+#pragma unroll
+        for (int k = 0; k < UNR; ++k) cur[k] = base + k < e1 ? ld_coherent(st.dist + tv[k]) : 0;
+#pragma unroll
+        for (int k = 0; k < UNR; ++k) {
+            int c = 0;
+            if (base + k < e1 && alt[k] < cur[k]) relax_tail<false>(st, pc, tv[k], alt[k], c);
+        }
+    }
+}
+
+// All lanes of `stride` workers (t = this one's index) walk the edges of one vertex.
+template <bool UNIT>
+__device__ __forceinline__ void relax_vertex(const DevState &st, const PushCtx &pc, int slot, int t, int stride) {
+    const int e0 = st.row[slot], e1 = st.row[slot + 1];
+    relax_range<UNIT, false, true>(st, pc, 0, e0, e1, ld_coherent(st.dist + slot), t, stride);
+}
+
+// Class prefix of the queued vertices of classes [c0, c1) into pre[0..NCLS] (thread 0).
+__device__ __forceinline__ void sol_prefix(const int *cnt, int c0, int c1, long long *pre) {
+    long long acc = 0;
+    for (int c = 0; c < NCLS; ++c) {
+        pre[c] = acc;
+        if (c >= c0 && c < c1) acc += cnt[c];
+    }
+    pre[NCLS] = acc;
+}
+
+struct SolBufs {
+    int *buf;  // global delayed buffer (solution 3): heavy vertices of one expansion
+    int *bcnt; // [3] its fill counters, rotating by iteration
+};
+
+enum SolMode { SOL_THREAD = 0, SOL_DUAL = 1, SOL_LDS = 2, SOL_GLOBAL = 3, SOL_WAVE = 4, SOL_WGLIST = 5 };
+
+// The parent kernel of solutions 0-5: one lane per queued vertex.
+//   SOL_THREAD  every vertex walked by its lane (solution 0, unorder_threadQueue_kernel)
+//   SOL_DUAL    light classes only; the heavy ones are the second queue (solution 1)
+//   SOL_LDS     heavy vertices of the workgroup's 256 into an LDS buffer, then the whole
+//               workgroup walks each one's edges (solution 2,
+//               unorder_threadQueue_shared_delaybuffer_kernel, sssp_kernel.cu:112-172)
+//   SOL_GLOBAL  heavy vertices appended to a global buffer (one atomic per wavefront),
+//               drained by k_sol_drain (solution 3, sssp_kernel.cu:179-209)
+//   SOL_WAVE    each wavefront then walks its own heavy vertices one by one with 64 lanes
+//               (solution 4: a 64-thread nested launch per heavy vertex, sssp_kernel.cu:237-286)
+//   SOL_WGLIST  heavy vertices of the workgroup into an LDS list; its four wavefronts take
+//               them in turn, 64 lanes per vertex (solution 5: one nested launch per block
+//               over the block's buffer, 64 threads per vertex, sssp_kernel.cu:289-327)
+template <int MODE, bool UNIT>
+__global__ __launch_bounds__(WG) void k_sol_parent(DevState st, SolBufs sb, int it_off) {
+    __shared__ long long s_pre[NCLS + 1];
+    __shared__ int s_seg[NCLS];
+    __shared__ int s_list[WG];
+    __shared__ int s_nlist;
+    Ctrl *ctrl = st.ctrl;
+    if (ld_coherent(&ctrl->done) || ld_coherent(&ctrl->error) || ld_coherent(&ctrl->skip)) return;
+    const int it = ld_coherent(&ctrl->it_base) + it_off;
+    const int hc = ld_coherent(&ctrl->cfg_klo) + 1;  // first heavy class
+    const int *cnt = st.cnt + (it % 3) * NCLS;
+    if (MODE == SOL_GLOBAL && blockIdx.x == 0 && threadIdx.x == 0) st_coherent(sb.bcnt + (it + 1) % 3, 0);
+    if (threadIdx.x < NCLS) s_seg[threadIdx.x] = st.seg_off[threadIdx.x];
+    if (threadIdx.x == 0) {
+        int lc[NCLS];
+        for (int c = 0; c < NCLS; ++c) lc[c] = ld_coherent(cnt + c);
+        sol_prefix(lc, 1, MODE == SOL_DUAL ? hc : NCLS, s_pre);
+        s_nlist = 0;
+    }
+    __syncthreads();
+    const long long total = s_pre[NCLS];
+    const int *q = st.queue[it & 1];
+    PushCtx pc;
+    pc.bm_next = st.bm[0];
+    const int lane = lane_id(), wave = threadIdx.x >> 6;
+    for (long long base = (long long)blockIdx.x * WG; base < total; base += (long long)gridDim.x * WG) {
+        const long long i = base + threadIdx.x;
+        int slot = 0, c = 0;
+        const bool valid = i < total;
+        if (valid) {
+            c = find_class(s_pre, i);
+            slot = ld_coherent(q + s_seg[c] + (i - s_pre[c]));
+        }
+        const bool heavy = valid && MODE != SOL_THREAD && MODE != SOL_DUAL && c >= hc;
+        if (valid && !heavy) relax_serial<UNIT>(st, pc, slot);
+        if (MODE == SOL_LDS || MODE == SOL_WGLIST) {
+            if (heavy) s_list[atomicAdd(&s_nlist, 1)] = slot;
+            __syncthreads();
+            const int nl = s_nlist;
+            if (MODE == SOL_LDS) {
+                for (int k = 0; k < nl; ++k) relax_vertex<UNIT>(st, pc, s_list[k], threadIdx.x, WG);
+            } else {
+                for (int k = wave; k < nl; k += WAVES) relax_vertex<UNIT>(st, pc, s_list[k], lane, 64);
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) s_nlist = 0;
+            __syncthreads();
+        } else if (MODE == SOL_WAVE) {
+            unsigned long long m = __ballot(heavy);
+            while (m) {
+                const int l = __ffsll((long long)m) - 1;
+                m &= m - 1;
+                relax_vertex<UNIT>(st, pc, __shfl(slot, l), lane, 64);
+            }
+        } else if (MODE == SOL_GLOBAL) {
+            const unsigned long long m = __ballot(heavy);
+            if (m) {
+                int at = 0;
+                if (lane == 0) at = atomicAdd(sb.bcnt + it % 3, __popcll(m));
+                at = __shfl(at, 0);
+                if (heavy) st_coherent(sb.buf + at + __popcll(m & __lanemask_lt()), slot);
+            }
+        }
+    }
+}
+
+// Second launch of solution 1: one workgroup per heavy vertex (unorder_blockQueue_kernel,
+// sssp_kernel.cu:57-78, over the heavy queue).
+template <bool UNIT>
+__global__ __launch_bounds__(WG) void k_sol_heavy(DevState st, int it_off) {
+    __shared__ long long s_pre[NCLS + 1];
+    __shared__ int s_seg[NCLS];
+    Ctrl *ctrl = st.ctrl;
+    if (ld_coherent(&ctrl->done) || ld_coherent(&ctrl->error) || ld_coherent(&ctrl->skip)) return;
+    const int it = ld_coherent(&ctrl->it_base) + it_off;
+    const int hc = ld_coherent(&ctrl->cfg_klo) + 1;
+    const int *cnt = st.cnt + (it % 3) * NCLS;
+    if (threadIdx.x < NCLS) s_seg[threadIdx.x] = st.seg_off[threadIdx.x];
+    if (threadIdx.x == 0) {
+        int lc[NCLS];
+        for (int c = 0; c < NCLS; ++c) lc[c] = ld_coherent(cnt + c);
+        sol_prefix(lc, hc, NCLS, s_pre);
+    }
+    __syncthreads();
+    const long long total = s_pre[NCLS];
+    const int *q = st.queue[it & 1];
+    PushCtx pc;
+    pc.bm_next = st.bm[0];
+    for (long long k = blockIdx.x; k < total; k += gridDim.x) {
+        const int c = find_class(s_pre, k);
+        relax_vertex<UNIT>(st, pc, ld_coherent(q + s_seg[c] + (k - s_pre[c])), threadIdx.x, WG);
+    }
+}
+
+// Second launch of solution 3: one wavefront per buffered vertex (NESTED_BLOCK_SIZE 64 threads
+// of sssp_process_buffer / unorder_blockQueue_kernel).
+template <bool UNIT>
+__global__ __launch_bounds__(WG) void k_sol_drain(DevState st, SolBufs sb, int it_off) {
+    Ctrl *ctrl = st.ctrl;
+    if (ld_coherent(&ctrl->done) || ld_coherent(&ctrl->error) || ld_coherent(&ctrl->skip)) return;
+    const int it = ld_coherent(&ctrl->it_base) + it_off;
+    const int nb = ld_coherent(sb.bcnt + it % 3);
+    PushCtx pc;
+    pc.bm_next = st.bm[0];
+    const long long nw = (long long)gridDim.x * WAVES;
+    for (long long k = (long long)blockIdx.x * WAVES + (threadIdx.x >> 6); k < nb; k += nw)
+        relax_vertex<UNIT>(st, pc, ld_coherent(sb.buf + k), lane_id(), 64);
+}
+
+// BFS_REC_GPU solution 0 (bfs_kernel_flat, bfs_rec_kernel.cu:41-58): level-synchronous and
+// topology-driven -- every vertex is inspected at every level; those at `level` set their
+// unreached or deeper children to level + 1 (plain stores: every writer of a level writes the
+// same value) and raise *changed.
+__global__ __launch_bounds__(WG) void k_bfs_flat(const int *row, const int *col, int *lvl, int n, int level,
+                                                 int *changed) {
+    int any = 0;
+    for (long long v = (long long)blockIdx.x * WG + threadIdx.x; v < n; v += (long long)gridDim.x * WG) {
+        if (lvl[v] != level) continue;
+        for (int e = row[v], e1 = row[v + 1]; e < e1; ++e) {
+            const int t = col[e];
+            if (lvl[t] > level + 1) { lvl[t] = level + 1; any = 1; }
+        }
+    }
+    if (__ballot(any) && lane_id() == 0) st_coherent(changed, 1);
+}
+
+// BFS_REC_GPU solution 2 (bfs_kernel_dp_hier, bfs_rec_kernel.cu:95-140): two levels per
+// expansion.  A queued vertex u's children are taken 64 at a time by one wavefront, a lane
+// per child (the reference's block per child); each child whose level this lane lowered to
+// level(u) + 1 is expanded at once by the whole wavefront (the reference's threads over the
+// grandchildren), and the grandchildren it lowers to level(u) + 2 are flagged for the next
+// expansion (the reference's nested launch per grandchild).  Levels are exact: a vertex is
+// expanded after every lowering of its level, as in the consolidated path.
+__device__ __forceinline__ int hier_chunks(int c) { return c <= 6 ? 1 : 1 << (c - 6); }
+
+__global__ __launch_bounds__(WG) void k_bfs_hier(DevState st, int it_off) {
+    __shared__ long long s_pre[NCLS + 1];
+    __shared__ int s_seg[NCLS];
+    Ctrl *ctrl = st.ctrl;
+    if (ld_coherent(&ctrl->done) || ld_coherent(&ctrl->error) || ld_coherent(&ctrl->skip)) return;
+    const int it = ld_coherent(&ctrl->it_base) + it_off;
+    const int *cnt = st.cnt + (it % 3) * NCLS;
+    if (threadIdx.x < NCLS) s_seg[threadIdx.x] = st.seg_off[threadIdx.x];
+    if (threadIdx.x == 0) {
+        long long acc = 0;
+        for (int c = 0; c < NCLS; ++c) {
+            s_pre[c] = acc;
+            if (c >= 1) acc += (long long)ld_coherent(cnt + c) * hier_chunks(c);
+        }
+        s_pre[NCLS] = acc;
+    }
+    __syncthreads();
+    const long long total = s_pre[NCLS];
+    const int *q = st.queue[it & 1];
+    PushCtx pc;
+    pc.bm_next = st.bm[0];
+    const int lane = lane_id();
+    const long long nw = (long long)gridDim.x * WAVES;
+    for (long long k = (long long)blockIdx.x * WAVES + (threadIdx.x >> 6); k < total; k += nw) {
+        const int c = find_class(s_pre, k);
+        const long long r = k - s_pre[c];
+        const int sh = c <= 6 ? 0 : c - 6;
+        const int u = ld_coherent(q + s_seg[c] + (r >> sh));
+        const int j = (int)(r & ((1ll << sh) - 1));
+        const long long e = (long long)st.row[u] + (long long)j * 64 + lane;
+        const int e1 = st.row[u + 1];
+        const int du = ld_coherent(st.dist + u);
+        bool won = false;
+        int child = 0;
+        if (e < e1) {
+            child = st.col[e];
+            const int alt = du + 1;
+            if (alt < ld_coherent(st.dist + child)) won = alt < atomicMin(&st.dist[child], alt);
+        }
+        unsigned long long m = __ballot(won);
+        while (m) {
+            const int l = __ffsll((long long)m) - 1;
+            m &= m - 1;
+            const int ch = __shfl(child, l);
+            relax_range<true, false, true>(st, pc, 0, st.row[ch], st.row[ch + 1], du + 1, lane, 64);
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------
 // source-packed batch: up to 32 sources of one graph solved together
 // ------------------------------------------------------------------------------------
 // Distances are vertex-major, 2^kshift sources per vertex row: dist[(v << kshift) + i].
@@ -1208,9 +1463,11 @@ __global__ __launch_bounds__(WG) void k_degree_class(const int *row, int n, unsi
 }
 
 // CSR well-formedness, so that no kernel can index outside the arrays:
-// row[0] == 0, row non-decreasing, row[n] == m, 0 <= col < n.  bad[0] counts violations.
-__global__ void k_validate_csr(const int *row, const int *col, int n, long long m, int *bad) {
-    int local = 0;
+// row[0] == 0, row non-decreasing, row[n] == m, 0 <= col < n.  out[0] counts violations;
+// with weights (w non-null) out[1] counts negative weights and out[2] receives the largest
+// weight (out[] starts at 0).
+__global__ void k_validate_csr(const int *row, const int *col, const int *w, int n, long long m, int *out) {
+    int local = 0, neg = 0, wmax = 0;
     const long long stride = (long long)gridDim.x * blockDim.x;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i <= n; i += stride) {
         int r = row[i];
@@ -1221,6 +1478,51 @@ __global__ void k_validate_csr(const int *row, const int *col, int n, long long 
     for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < m; e += stride) {
         int c = col[e];
         if (c < 0 || c >= n) local++;
+        if (w) {
+            int x = w[e];
+            if (x < 0) neg++;
+            wmax = max(wmax, x);
+        }
+    }
+    if (local) atomicAdd(&out[0], local);
+    if (neg) atomicAdd(&out[1], neg);
+    if (wmax) atomicMax(&out[2], wmax);
+}
+
+// Largest finite value of a solve's distance buffer (count entries): atomicMax into *out,
+// which starts at 0.
+__global__ __launch_bounds__(WG) void k_max_finite(const int *dist, long long count, int inf, int *out) {
+    int mx = 0;
+    for (long long i = (long long)blockIdx.x * WG + threadIdx.x; i < count; i += (long long)gridDim.x * WG) {
+        int d = dist[i];
+        if (d != inf) mx = max(mx, d);
+    }
+    for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o));
+    if (lane_id() == 0 && mx) atomicMax(out, mx);
+}
+
+// Relaxations that int32 distances below inf cannot hold: an edge u -> v whose tail is
+// reached, whose target is still at inf and with dist[u] + w >= inf (the relaxation
+// alt < dist[v] is false there, so the target stays unreached although a path exists).
+// Counts them into *bad.  Every layout: one distance per slot (BATCH: slots of
+// concatenated instances) or rows of 2^kshift sources per vertex.
+template <bool BATCH>
+__global__ __launch_bounds__(WG) void k_dist_escape(DevState st, int *bad) {
+    const int ks = st.kshift < 0 ? 0 : st.kshift;
+    const long long total = st.nslots << ks;
+    int local = 0;
+    for (long long i = (long long)blockIdx.x * WG + threadIdx.x; i < total; i += (long long)gridDim.x * WG) {
+        const int du = st.dist[i];
+        if (du == st.inf) continue;
+        const int slot = (int)(i >> ks);
+        const int lane = (int)(i & ((1 << ks) - 1));
+        int B = 0, VB = 0;
+        decode<BATCH>(st, slot, B, VB);
+        const int v = slot - B;
+        for (int e = st.row[VB + v], e1 = st.row[VB + v + 1]; e < e1; ++e) {
+            const long long t = (long long)(B + st.col[e]);
+            if (st.dist[(t << ks) + lane] == st.inf && (long long)du + st.w[e] >= (long long)st.inf) local++;
+        }
     }
     if (local) atomicAdd(bad, local);
 }
@@ -1421,11 +1723,30 @@ hipError_t launch_fill_u32(unsigned *p, long long n, unsigned v, hipStream_t s) 
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n, v);
     return hipGetLastError();
 }
-hipError_t launch_validate_csr(const int *row, const int *col, int n, long long m, int *bad,
+hipError_t launch_validate_csr(const int *row, const int *col, const int *w, int n, long long m, int *out3,
                                hipStream_t s) {
     long long work = (long long)n + 1 > m ? (long long)n + 1 : m;
     hipLaunchKernelGGL(k_validate_csr, dim3(grid_for(work, 256) > 4096 ? 4096 : grid_for(work, 256)),
-                       dim3(256), 0, s, row, col, n, m, bad);
+                       dim3(256), 0, s, row, col, w, n, m, out3);
+    return hipGetLastError();
+}
+
+hipError_t launch_max_finite(const DevState &st, int *out, hipStream_t s) {
+    const long long count = st.nslots << (st.kshift < 0 ? 0 : st.kshift);
+    hipError_t e = launch_fill_i32(out, 1, 0, s);
+    if (e != hipSuccess || count <= 0) return e;
+    hipLaunchKernelGGL(k_max_finite, dim3(grid_for(count, WG) > 4096 ? 4096 : grid_for(count, WG)), dim3(WG),
+                       0, s, (const int *)st.dist, count, st.inf, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_dist_escape(const DevState &st, bool batch, int *bad, hipStream_t s) {
+    const long long count = st.nslots << (st.kshift < 0 ? 0 : st.kshift);
+    hipError_t e = launch_fill_i32(bad, 1, 0, s);
+    if (e != hipSuccess || count <= 0) return e;
+    const dim3 grid(grid_for(count, WG) > 4096 ? 4096 : grid_for(count, WG));
+    if (batch) hipLaunchKernelGGL(k_dist_escape<true>, grid, dim3(WG), 0, s, st, bad);
+    else hipLaunchKernelGGL(k_dist_escape<false>, grid, dim3(WG), 0, s, st, bad);
     return hipGetLastError();
 }
 hipError_t launch_degree_class(const int *row, int n, unsigned char *cls, int *hist, hipStream_t s) {
@@ -1441,6 +1762,7 @@ hipError_t launch_init_solve(const DevState &st, const int *d_src_slots, int nsr
     if ((e = launch_fill_u32(st.bm[0], st.nwords, 0u, s)) != hipSuccess) return e;
     if ((e = launch_fill_u32(st.bm[1], st.nwords, 0u, s)) != hipSuccess) return e;
     if ((e = launch_fill_i32(st.cnt, 3 * NCLS, 0, s)) != hipSuccess) return e;
+    if (st.sbcnt && (e = launch_fill_i32(st.sbcnt, 3, 0, s)) != hipSuccess) return e;
     if ((e = launch_fill_i32((int *)st.ctrl, (long long)(sizeof(Ctrl) / sizeof(int)), 0, s)) != hipSuccess)
         return e;
     if (push) hipLaunchKernelGGL(k_init_sources<true>, dim3(1), dim3(256), 0, s, st, d_src_slots, nsrc, T0);
@@ -1518,6 +1840,7 @@ hipError_t launch_init_packed(const DevState &st, const int *d_srcs, int nsrc, i
     if ((e = launch_fill_u32(st.bm[0], st.nwords, 0u, s)) != hipSuccess) return e;
     if ((e = launch_fill_u32(st.mflag, st.nslots, 0u, s)) != hipSuccess) return e;
     if ((e = launch_fill_i32(st.cnt, 3 * NCLS, 0, s)) != hipSuccess) return e;
+    if (st.sbcnt && (e = launch_fill_i32(st.sbcnt, 3, 0, s)) != hipSuccess) return e;
     if ((e = launch_fill_i32((int *)st.ctrl, (long long)(sizeof(Ctrl) / sizeof(int)), 0, s)) != hipSuccess)
         return e;
     hipLaunchKernelGGL(k_pk_init_sources, dim3(1), dim3(64), 0, s, st, d_srcs, nsrc, delta > 0 ? delta : NO_WINDOW);
@@ -1533,8 +1856,43 @@ hipError_t launch_decide(const DevState &st, const SchedDev &sp, int it_off, hip
     hipLaunchKernelGGL(k_decide, dim3(1), dim3(WG), 0, s, st, sp, it_off);
     return hipGetLastError();
 }
+// The expansion of the reference's other schedules (single-source, scan frontier).
+static hipError_t launch_scheme(const DevState &st, const SchedDev &sp, int it_off, bool unit, int wgs,
+                                hipStream_t s) {
+    const SolBufs sb{st.sbuf, st.sbcnt};
+#define NP_SOL(M) do { if (unit) hipLaunchKernelGGL((k_sol_parent<M, true>), dim3(wgs), dim3(WG), 0, s, st, sb, it_off); \
+                       else hipLaunchKernelGGL((k_sol_parent<M, false>), dim3(wgs), dim3(WG), 0, s, st, sb, it_off); } while (0)
+    switch (sp.scheme) {
+    case NP_SCHEME_THREAD: NP_SOL(SOL_THREAD); break;
+    case NP_SCHEME_DUAL_QUEUE:
+        NP_SOL(SOL_DUAL);
+        if (unit) hipLaunchKernelGGL(k_sol_heavy<true>, dim3(wgs), dim3(WG), 0, s, st, it_off);
+        else hipLaunchKernelGGL(k_sol_heavy<false>, dim3(wgs), dim3(WG), 0, s, st, it_off);
+        break;
+    case NP_SCHEME_LDS_BUFFER: NP_SOL(SOL_LDS); break;
+    case NP_SCHEME_GLOBAL_BUFFER:
+        NP_SOL(SOL_GLOBAL);
+        if (unit) hipLaunchKernelGGL(k_sol_drain<true>, dim3(wgs), dim3(WG), 0, s, st, sb, it_off);
+        else hipLaunchKernelGGL(k_sol_drain<false>, dim3(wgs), dim3(WG), 0, s, st, sb, it_off);
+        break;
+    case NP_SCHEME_WAVE_NESTED: NP_SOL(SOL_WAVE); break;
+    case NP_SCHEME_WG_NESTED: NP_SOL(SOL_WGLIST); break;
+    case NP_SCHEME_BFS_HIER: hipLaunchKernelGGL(k_bfs_hier, dim3(wgs), dim3(WG), 0, s, st, it_off); break;
+    default: return hipErrorInvalidValue;
+    }
+#undef NP_SOL
+    return hipGetLastError();
+}
+
+hipError_t launch_bfs_flat(const DevState &st, int level, int *changed, int wgs, hipStream_t s) {
+    hipLaunchKernelGGL(k_bfs_flat, dim3(wgs), dim3(WG), 0, s, st.row, st.col, st.dist, (int)st.nslots, level,
+                       changed);
+    return hipGetLastError();
+}
+
 hipError_t launch_expand(const DevState &st, const SchedDev &sp, int it_off, bool batch, bool unit,
                          int wgs, hipStream_t s) {
+    if (sp.scheme != NP_SCHEME_CONSOLIDATE) return launch_scheme(st, sp, it_off, unit, wgs, s);
     const bool push = sp.push != 0;
     if (st.kshift >= 0) {
         if (unit) hipLaunchKernelGGL(k_pk_expand<true>, dim3(wgs), dim3(WG), 0, s, st, it_off);

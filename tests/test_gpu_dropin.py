@@ -101,3 +101,39 @@ def test_cli_beside_a_process_holding_the_cooperative_queue(tmp_path, npa, ctx, 
     lines = out.read_text().splitlines()
     assert lines == ["From Node %d -> %d is %d" % (src + 1, i + 1, x) for i, x in enumerate(g["dist"][k])]
     G.free()
+
+
+REF_BIN = os.path.join(os.path.dirname(PKG_DIR), "oracle", "_ref")
+
+
+def _ref_driver(name):
+    p = os.path.join(REF_BIN, "ref-gpu-" + name)
+    if not os.access(p, os.X_OK):
+        pytest.skip("relinked reference driver %s not built (oracle/Makefile needs the reference sources)" % name)
+    return p
+
+
+@pytest.mark.parametrize("name", ["d9_rand", "d10_small", "slndc_small"])
+@pytest.mark.parametrize("solution", ["0", "1", "2", "3"])
+def test_reference_bfs_rec_main_relinked(tmp_path, name, solution):
+    """The reference's own bfs_rec.cpp main(), unchanged, linked against libnpdropin.so: it runs
+    BFS_REC_GPU N=10 times and checks the levels with its own bfs_cpu + validateArrays."""
+    exe = _ref_driver("bfs_rec")
+    g = load_golden(name + ".npz")
+    r = subprocess.run([exe, "-f", str(int(g["fmt"])), "-i", os.path.join(GOLDEN, str(g["path"])), "-s", solution],
+                       cwd=tmp_path, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "PASS: GPU bfs rec !" in r.stdout, r.stdout
+
+
+@pytest.mark.parametrize("solution", [str(s) for s in range(7)])
+def test_reference_sssp_main_relinked(tmp_path, solution):
+    """The reference's own sssp.cpp main(), unchanged, linked against libnpdropin.so: reads the
+    DIMACS9 file, builds the CSR and calls SSSP_GPU N=10 times with the given solution."""
+    exe = _ref_driver("sssp")
+    r = subprocess.run([exe, "-f", "0", "-i", os.path.join(GOLDEN, "graphs", "d9_rand.gr"), "-s", solution, "-v"],
+                       cwd=tmp_path, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "CUDA kernel execution" in r.stdout
+    if solution == "6":
+        assert r.stdout.count("iteration:") == 10
