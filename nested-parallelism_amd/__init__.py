@@ -70,7 +70,8 @@ class Sched(ctypes.Structure):
 class Stats(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int32), ("status", ctypes.c_int32), ("kernel_ms", ctypes.c_double),
                 ("frontier_total", ctypes.c_int64), ("drive", ctypes.c_int32), ("pad", ctypes.c_int32),
-                ("pairs_total", ctypes.c_int64), ("reserved", ctypes.c_int64 * 4)]
+                ("pairs_total", ctypes.c_int64), ("edges_total", ctypes.c_int64),
+                ("items_total", ctypes.c_int64), ("reserved", ctypes.c_int64 * 2)]
 
     def __repr__(self):
         return "Stats(iterations=%d, kernel_ms=%.4f, frontier_total=%d)" % (
@@ -80,7 +81,7 @@ class Stats(ctypes.Structure):
 class TraceRec(ctypes.Structure):
     _fields_ = [("iteration", ctypes.c_int32), ("level", ctypes.c_int32), ("log2_t_lo", ctypes.c_int32),
                 ("log2_t_hi", ctypes.c_int32), ("count", ctypes.c_int32 * NCLASS), ("t_begin", ctypes.c_int64),
-                ("t_end", ctypes.c_int64)]
+                ("t_end", ctypes.c_int64), ("pairs", ctypes.c_int64), ("edges", ctypes.c_int64)]
 
 
 _lib = None

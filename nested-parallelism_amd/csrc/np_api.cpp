@@ -464,6 +464,8 @@ static int run_drive(np_ctx *c, Space &s, const DevState &st, const np_sched &sc
         stats->drive = drive;
         stats->pairs_total = st.kshift >= 0 ? h.pairs_total : h.frontier_total;
         stats->frontier_total = h.frontier_total;
+        stats->edges_total = h.edges_total;
+        stats->items_total = h.items_total;
     }
     return rc;
 }
@@ -1614,6 +1616,8 @@ static int run_parts_concurrent(np_ctx *c, std::vector<PartJob> &jobs, const np_
         acc->iterations += h.iters;
         acc->frontier_total += h.frontier_total;
         acc->pairs_total += j.st.kshift >= 0 ? h.pairs_total : h.frontier_total;
+        acc->edges_total += h.edges_total;
+        acc->items_total += h.items_total;
         if (h.error && rc == NP_OK) rc = h.error;
     }
     if (rc == NP_ERR_NOT_CONVERGED) set_error("iteration cap reached");
@@ -1643,6 +1647,8 @@ int np_batch_solve(np_ctx *c, np_batch *b, const np_sched *sched, np_stats *stat
         acc.kernel_ms += p.kernel_ms;
         acc.frontier_total += p.frontier_total;
         acc.pairs_total += p.pairs_total;
+        acc.edges_total += p.edges_total;
+        acc.items_total += p.items_total;
         if (acc.drive < 0) acc.drive = p.drive;
     };
     // packs: scan frontier
@@ -1659,6 +1665,7 @@ int np_batch_solve(np_ctx *c, np_batch *b, const np_sched *sched, np_stats *stat
         st.kshift = pk->kshift;
         st.mflag = s.mflag;
         st.qmask[0] = s.qmask[0]; st.qmask[1] = s.qmask[1];
+        st.count_work = sch.trace ? 1 : 0;
         SchedDev sd = make_schedd(c, psch, g->n);
         if (psch.max_iters <= 0) {
             // Without a window every source's frontier empties within n+1 expansions; with

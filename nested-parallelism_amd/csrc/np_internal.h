@@ -27,6 +27,8 @@ struct Ctrl {
     int defer_cnt[3];         // per compaction pass (rotating): flagged vertices kept back
     int defer_min[3];         // ... and the least distance among them
     long long pairs_total;    // source-packed: sum over iterations of queued (vertex, source) pairs
+    long long edges_total;    // source-packed: sum over iterations of the queued vertices' out-degrees
+    long long items_total;    // source-packed: ... of out-degree x queued sources ((edge, source) items)
 };
 
 // Everything an expansion / compaction touches.  Indices are "slots": the vertex ids of
@@ -57,6 +59,9 @@ struct DevState {
     // counters [3], rotating by iteration
     int *sbuf;
     int *sbcnt;
+    // source-packed: accumulate Ctrl::edges_total / items_total in the compaction (one row
+    // read per queued vertex; set when the solve records a trace)
+    int count_work;
 };
 
 // Largest number of sources packed into one distance row.

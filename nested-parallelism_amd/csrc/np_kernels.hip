@@ -501,9 +501,11 @@ __device__ __noinline__ Cfg decide(const SchedDev &sp, const int *cnt, int it) {
 }
 
 __device__ __noinline__ void record_trace(const SchedDev &sp, int it, const Cfg &cfg, const int *cnt,
-                             long long t0) {
+                             long long t0, Ctrl *ctrl) {
     if (!sp.trace || it >= sp.trace_cap) return;
     np_trace_rec *r = sp.trace + it;
+    r->pairs = __hip_atomic_load(&ctrl->pairs_total, __ATOMIC_RELAXED, NP_AGENT);
+    r->edges = __hip_atomic_load(&ctrl->edges_total, __ATOMIC_RELAXED, NP_AGENT);
     r->iteration = it;
     r->level = cfg.level;
     r->log2_t_lo = cfg.klo;
@@ -579,7 +581,7 @@ __global__ __launch_bounds__(WG) void k_persistent(DevState st, SchedDev sp) {
         if (threadIdx.x == 0) build_tables(tb, cfg);
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             ctrl->frontier_total += total;
-            record_trace(sp, it, cfg, tb.cnt, t_prev);
+            record_trace(sp, it, cfg, tb.cnt, t_prev, ctrl);
         }
         __syncthreads();
         PushCtx pc;
@@ -650,7 +652,7 @@ __global__ __launch_bounds__(WG) void k_decide(DevState st, SchedDev sp, int it_
             cfg = make_cfg(ctrl->cfg_level, ctrl->cfg_klo, ctrl->cfg_khi);
         }
         ctrl->frontier_total += tot;
-        record_trace(sp, ctrl->iters, cfg, s_cnt, (long long)wall_clock64());
+        record_trace(sp, ctrl->iters, cfg, s_cnt, (long long)wall_clock64(), ctrl);
         ctrl->iters += 1;
         ctrl->skip = 0;
         if (!sp.push) ctrl->pass = pass + 1;
@@ -1219,6 +1221,7 @@ __device__ __forceinline__ void pk_compaction(const DevState &st, int *cnt_out, 
     if (threadIdx.x < NCLS) s_lc[threadIdx.x] = 0;
     __syncthreads();
     int pairs = 0, my_dc = 0, my_dm = NO_WINDOW;
+    long long edges = 0, items = 0;
     const long long nv = st.nwords * 32;
     // Without a window every pending vertex is taken whole and nothing stays flagged: the frontier
     // is as sparse as a single source's, so a thread scans one bitmap word as compaction_impl does.
@@ -1244,6 +1247,11 @@ __device__ __forceinline__ void pk_compaction(const DevState &st, int *cnt_out, 
             st_coherent(q_out + qi, (int)v);
             st_coherent_u(qm_out + qi, t);
             pairs += __popc(t);
+            if (st.count_work) {
+                const long long d = st.row[v + 1] - st.row[v];
+                edges += d;
+                items += d * __popc(t);
+            }
         }
         if (word) st_coherent_u(bm + wi, 0u);
         __syncthreads();
@@ -1288,14 +1296,31 @@ __device__ __forceinline__ void pk_compaction(const DevState &st, int *cnt_out, 
         for (int k = 0; k < CPT; ++k) {
             if (!qc[k]) continue;
             const long long qi = st.seg_off[qc[k]] + s_lb[qc[k]] + qp[k];
-            st_coherent(q_out + qi, (int)(vb + k * WG + threadIdx.x));
+            const long long v = vb + k * WG + threadIdx.x;
+            st_coherent(q_out + qi, (int)v);
             st_coherent_u(qm_out + qi, qt[k]);
             pairs += __popc(qt[k]);
+            if (st.count_work) {
+                const long long d = st.row[v + 1] - st.row[v];
+                edges += d;
+                items += d * __popc(qt[k]);
+            }
         }
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) pairs += __shfl_xor(pairs, o);
     if (lane_id() == 0 && pairs) atomicAdd((unsigned long long *)&st.ctrl->pairs_total, (unsigned long long)pairs);
+    if (st.count_work) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            edges += __shfl_xor(edges, o);
+            items += __shfl_xor(items, o);
+        }
+        if (lane_id() == 0 && edges) {
+            atomicAdd((unsigned long long *)&st.ctrl->edges_total, (unsigned long long)edges);
+            atomicAdd((unsigned long long *)&st.ctrl->items_total, (unsigned long long)items);
+        }
+    }
     if (T != NO_WINDOW) flush_deferred(my_dc, my_dm, &s_dc, &s_dm, defer_cnt, defer_min);
 }
 
@@ -1367,7 +1392,7 @@ __global__ __launch_bounds__(WG) void k_pk_persistent(DevState st, SchedDev sp) 
         if (threadIdx.x == 0) build_tables(tb, pk_cfg(cfg));
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             ctrl->frontier_total += total;
-            record_trace(sp, it, cfg, tb.cnt, t_prev);
+            record_trace(sp, it, cfg, tb.cnt, t_prev, ctrl);
         }
         __syncthreads();
         pk_expansion<UNIT, false>(st, q_cur, qm_cur, tb);

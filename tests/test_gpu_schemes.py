@@ -71,10 +71,12 @@ def test_schemes_high_diameter_and_edge_cases(npa, ctx, orc):
     G = npa.Graph.grid(ctx, 64, 48, None, 1, 1000, seed=62)
     row, col, w = G.download()
     ref, _ = orc.dijkstra(row, col, w, 100)
+    # every reached vertex with out-edges is queued at least once (the source included)
+    queued_min = int(np.count_nonzero((ref < INF) & (np.diff(row) > 0)))
     for sc in SSSP_SCHEMES:
         d, st = ctx.sssp(G, 100, _scheme(npa, sc))
         assert np.array_equal(d, ref), sc
-        assert st.iterations > 50
+        assert st.iterations > 1 and st.frontier_total >= queued_min, (sc, st)
     for sc in BFS_SCHEMES:
         lev, st = ctx.bfs(G, 100, _scheme(npa, sc))
         assert np.array_equal(lev, orc.bfs_levels(row, col, 100)), sc
