@@ -468,20 +468,29 @@ def main():
 
     npa = load_pkg()
     ctx = npa.Context(0)
+    try:
+        rc = run_config(npa, ctx, args)
+    finally:
+        ctx.close()  # every device object is released before the interpreter exits
+    if rc:
+        sys.exit(rc)
+
+
+def run_config(npa, ctx, args):
     ncu, occ = ctx.device_info()
     log("device: %d CUs, persistent occupancy %d workgroups/CU" % (ncu, occ))
     if args.config == "bc":
         bc_bench(npa, ctx, args)
-        return
+        return 0
     if args.config == "color":
         color_bench(npa, ctx, args)
-        return
+        return 0
     if args.config == "pagerank":
         pr_bench(npa, ctx, args)
-        return
+        return 0
     if args.config == "spmv":
         spmv_bench(npa, ctx, args)
-        return
+        return 0
     t0 = time.time()
     W = build_workload(npa, ctx, args.config, args)
     log("%s: %d graphs, %d instances, built in %.1fs" % (W.desc, len(W.graphs), len(W.inst), time.time() - t0))
@@ -662,9 +671,7 @@ def main():
     dbuf.free()
     for g in W.graphs:
         g.free()
-    ctx.close()
-    if exact is False:
-        sys.exit(1)
+    return 1 if exact is False else 0
 
 
 if __name__ == "__main__":

@@ -13,8 +13,10 @@ bfs_rec_wrapper.cu:231-269); here the same solves take the CSR explicitly:
 There is no CPU fallback: every solve runs the HIP kernels, and a missing library or
 device raises.
 """
+import atexit
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -326,6 +328,22 @@ def features(count, it, n):
 # ---------------------------------------------------------------------------------------
 # device objects
 # ---------------------------------------------------------------------------------------
+_LIVE = weakref.WeakSet()
+
+
+@atexit.register
+def _close_live_contexts():
+    """Contexts still open when the interpreter exits are destroyed here, while the HIP
+    runtime (and any tool attached to it, such as a profiler) is still fully up: no HIP
+    call of this library is left for the C runtime's exit handlers or for object
+    finalisation during interpreter teardown."""
+    for c in list(_LIVE):
+        try:
+            c.close()
+        except Exception:
+            pass
+
+
 class Context:
     """np_ctx: one HIP device and one stream (sssp_wrapper.cu:58-140 prepare_gpu's device)."""
 
@@ -335,6 +353,7 @@ class Context:
         _check(L.np_create(device, ctypes.byref(h)))
         self.h = h
         self.device = device
+        _LIVE.add(self)
 
     def close(self):
         if self.h:
