@@ -223,6 +223,20 @@ int np_sssp(np_ctx *ctx, np_graph *g, int32_t src, const np_sched *sched, int32_
  * host and the call returns once the solve has finished on the device. */
 int np_sssp_device(np_ctx *ctx, np_graph *g, int32_t src, const np_sched *sched, int32_t *d_dist,
                    np_stats *stats);
+/* np_sssp_device without any device allocation: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (tight serving loops, harnesses that forbid
+ * hipMalloc).  The solver state (bitmaps, frontier queues, counters, the renumbered
+ * graph's distances) lives in a file-scope device array of the library sized for graphs of
+ * at most NP_NOALLOC_MAX_VERTICES vertices (about 32.5 MiB, held by every GPU the library
+ * runs on until the process ends); larger graphs fail with NP_ERR_CAPACITY.  The graph
+ * must have been created on ctx (its CSR and degree classes are the caller's inputs).
+ * Results are identical to np_sssp_device's.  The graph drive runs as the host loop
+ * (capturing a graph allocates); stats.drive says which drive ran.  Every call on one GPU
+ * shares that GPU's array, which keeps the last call's contents: calls that overlap on
+ * one GPU (two contexts, two threads) race and must be serialised by the caller. */
+#define NP_NOALLOC_MAX_VERTICES 2097152
+int np_sssp_device_noalloc(np_ctx *ctx, np_graph *g, int32_t src, const np_sched *sched, int32_t *d_dist,
+                           np_stats *stats);
 /* Recursive BFS levels (BFS_REC_GPU with solution 3): unit weights, level[src] = 0,
  * unreached = NP_UNDEFINED.  level_out: host int32[n]. */
 int np_bfs(np_ctx *ctx, np_graph *g, int32_t src, const np_sched *sched, int32_t *level_out,

@@ -30,6 +30,7 @@ MLP_NIN = 10
 CTX_RELABEL = 1           # np_set_option key: 0 caller ids, 1 always renumber, 2 when skewed
 CTX_PACK = 2              # np_set_option key: 1 batch instances sharing a graph are source-packed
 MLP_NPARAM = 32 * 10 + 32 + 32 * 32 + 32 + 5 * 32 + 5
+NOALLOC_MAX_VERTICES = 2097152  # np_sssp_device_noalloc's largest graph
 
 LEVEL_WAVE, LEVEL_WORKGROUP, LEVEL_GRID, LEVEL_LEARNED = 0, 1, 2, -1
 DRIVE_PERSISTENT, DRIVE_HOSTLOOP, DRIVE_GRAPH = 0, 1, 2
@@ -118,6 +119,8 @@ _SIGS = [
     ("np_sssp", ctypes.c_int, [_vp, _vp, ctypes.c_int32, ctypes.POINTER(Sched), _vp, ctypes.POINTER(Stats)]),
     ("np_sssp_device", ctypes.c_int, [_vp, _vp, ctypes.c_int32, ctypes.POINTER(Sched), _vp,
                                       ctypes.POINTER(Stats)]),
+    ("np_sssp_device_noalloc", ctypes.c_int, [_vp, _vp, ctypes.c_int32, ctypes.POINTER(Sched), _vp,
+                                              ctypes.POINTER(Stats)]),
     ("np_bfs", ctypes.c_int, [_vp, _vp, ctypes.c_int32, ctypes.POINTER(Sched), _vp, ctypes.POINTER(Stats)]),
     ("np_batch_create", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.POINTER(_vp), _vp, ctypes.c_int32,
                                        ctypes.POINTER(_vp)]),
@@ -395,6 +398,14 @@ class Context:
         st = Stats()
         _check(load().np_sssp_device(self.h, g.h, int(src), ctypes.byref(sched or Sched()),
                                      ctypes.c_void_p(d_dist_ptr), ctypes.byref(st)))
+        return st
+
+    def sssp_device_noalloc(self, g, src, d_dist_ptr, sched=None):
+        """sssp_device with no device allocation (np_sssp_device_noalloc): graphs of at most
+        NOALLOC_MAX_VERTICES vertices; calls overlapping on one GPU race."""
+        st = Stats()
+        _check(load().np_sssp_device_noalloc(self.h, g.h, int(src), ctypes.byref(sched or Sched()),
+                                             ctypes.c_void_p(d_dist_ptr), ctypes.byref(st)))
         return st
 
     def bfs(self, g, src, sched=None):

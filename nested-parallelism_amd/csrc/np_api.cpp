@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <string>
 #include <functional>
+#include <mutex>
 
 #include "np_internal.h"
 
@@ -151,6 +152,7 @@ struct np_ctx {
     std::vector<np_graph *> graphs;
     std::vector<np_batch *> batches;
     std::vector<void *> allocs;             // np_dev_alloc
+    unsigned char *scratch = nullptr;       // this device's copy of the static scratch array
 };
 
 template <class T>
@@ -655,6 +657,45 @@ extern "C" {
 const char *np_last_error(void) { return g_err; }
 const char *np_version(void) { return "np-sssp gfx950 1.0"; }
 
+// ------------------------------------------------------------------------------------
+// allocation-free single-source solves (np_sssp_device_noalloc)
+// ------------------------------------------------------------------------------------
+// Solver state of one solve on a graph of at most NOALLOC_MAX_N vertices, carved from a
+// file-scope device array: distances (renumbered graphs), two update bitmaps, two frontier
+// queues, the delayed buffer of NP_SCHEME_GLOBAL_BUFFER, class counters and offsets and the
+// source slot.  Every GPU the library runs on holds one copy until the process ends; calls
+// on one GPU share it, so calls that overlap on one GPU race.
+static constexpr long long NOALLOC_MAX_N = NP_NOALLOC_MAX_VERTICES;
+static constexpr size_t NA_ALIGN = 256;
+static constexpr size_t na_round(size_t b) { return (b + NA_ALIGN - 1) / NA_ALIGN * NA_ALIGN; }
+static constexpr size_t noalloc_bytes(long long n) {
+    return 4 * na_round(4 * (size_t)n)                    // dist, queue[0], queue[1], sbuf
+           + 2 * na_round(4 * (size_t)((n + 31) / 32))   // bm[0], bm[1]
+           + na_round(4 * 3 * NCLS) + na_round(4 * (NCLS + 1)) + na_round(4) + na_round(4 * 3);
+}
+static constexpr size_t NOALLOC_BYTES = noalloc_bytes(NOALLOC_MAX_N);
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+static constexpr int NA_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static unsigned char *g_scratch_ptr[NA_MAX_DEVICES] = {nullptr};
+
+// The device address of this device's `scratch` (the device must be current).  In host
+// code the array's name is only a host-side stand-in; kernels get this address.
+static int scratch_setup(int device, unsigned char **out) {
+    *out = nullptr;
+    if (device >= NA_MAX_DEVICES) return NP_OK;  // no allocation-free solves on such a device
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return hip_fail(e != hipSuccess ? e : hipErrorInvalidSymbol, "scratch symbol address", __LINE__);
+        g_scratch_ptr[device] = (unsigned char *)p;
+    }
+    *out = g_scratch_ptr[device];
+    return NP_OK;
+}
+
 int np_create(int device, np_ctx **out) {
     if (!out) { set_error("null out"); return NP_ERR_ARG; }
     *out = nullptr;
@@ -692,6 +733,8 @@ int np_create(int device, np_ctx **out) {
         return rc;
     }
     std::memset(c->h_ctrl, 0, sizeof(Ctrl));
+    int rc = scratch_setup(device, &c->scratch);
+    if (rc) { np_destroy(c); return rc; }
     *out = c;
     return NP_OK;
 }
@@ -908,6 +951,65 @@ int np_sssp(np_ctx *c, np_graph *g, int32_t src, const np_sched *sched, int32_t 
 int np_sssp_device(np_ctx *c, np_graph *g, int32_t src, const np_sched *sched, int32_t *d_dist, np_stats *stats) {
     if (!d_dist) { set_error("null d_dist"); return NP_ERR_ARG; }
     return solve_single(c, g, src, sched, d_dist, nullptr, false, stats);
+}
+
+int np_sssp_device_noalloc(np_ctx *c, np_graph *g, int32_t src, const np_sched *sched, int32_t *d_dist,
+                           np_stats *stats) {
+    if (!c || !g) { set_error("null context or graph"); return NP_ERR_ARG; }
+    if (!d_dist) { set_error("null d_dist"); return NP_ERR_ARG; }
+    if (g->ctx != c) { set_error("graph belongs to another context or its context was destroyed"); return NP_ERR_ARG; }
+    if (!c->scratch) { set_error("no scratch array on device %d", c->device); return NP_ERR_UNSUPPORTED; }
+    if ((long long)g->n > NOALLOC_MAX_N) {
+        set_error("graph of %d vertices exceeds the %lld vertices of the allocation-free solve's scratch", g->n,
+                  NOALLOC_MAX_N);
+        return NP_ERR_CAPACITY;
+    }
+    if (src < 0 || src >= g->n) { set_error("source %d out of range [0,%d)", src, g->n); return NP_ERR_ARG; }
+    HIPCHK(hipSetDevice(c->device));
+    np_sched sch;
+    int rc = validate_sched(sched, &sch);
+    if (rc) return rc;
+    if (sch.scheme == NP_SCHEME_BFS_FLAT || sch.scheme == NP_SCHEME_BFS_HIER) {
+        set_error("scheme %d computes BFS levels (np_bfs) only", sch.scheme);
+        return NP_ERR_ARG;
+    }
+    if (sch.drive == NP_DRIVE_GRAPH) sch.drive = NP_DRIVE_HOSTLOOP;  // capturing a graph allocates
+    // the solver state of graph_space(), carved from this device's scratch (nothing is freed:
+    // the local Space only names the pieces)
+    Space s;
+    s.nslots = g->n;
+    std::memcpy(s.hist, g->hist, sizeof(s.hist));
+    s.cls = g->cls;
+    set_guard(s, g->n, g->max_w);
+    int seg[NCLS + 1];
+    seg_from_hist(s.hist, seg, &s.qcap);
+    s.nwords = (s.nslots + 31) / 32;
+    unsigned char *at = c->scratch;
+    auto carve = [&](size_t bytes) { unsigned char *p = at; at += na_round(bytes); return p; };
+    const size_t n4 = 4 * (size_t)g->n;
+    s.dist = (int *)carve(n4);
+    s.queue[0] = (int *)carve(n4);
+    s.queue[1] = (int *)carve(n4);
+    s.sbuf = (int *)carve(n4);
+    s.bm[0] = (unsigned *)carve(4 * (size_t)s.nwords);
+    s.bm[1] = (unsigned *)carve(4 * (size_t)s.nwords);
+    s.cnt = (int *)carve(4 * 3 * NCLS);
+    s.seg_off = (int *)carve(4 * (NCLS + 1));
+    s.src_slots = (int *)carve(4);
+    s.sbcnt = (int *)carve(4 * 3);
+    if ((size_t)(at - c->scratch) > NOALLOC_BYTES) { set_error("scratch layout exceeds its array"); return NP_ERR_CAPACITY; }
+    const int ssrc = g->solver_id(src, c->stream);
+    if (ssrc < 0) { set_error("source mapping failed"); return NP_ERR_HIP; }
+    int *dist = g->rl.active ? s.dist : d_dist;
+    DevState st = make_state(c, s, g->srow, g->scol, g->sw, dist, INF);
+    SchedDev sd = make_schedd(c, sch, g->n);
+    HIPCHK(hipMemcpyAsync(s.seg_off, seg, sizeof(seg), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(s.src_slots, &ssrc, sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(launch_init_solve(st, s.src_slots, 1, sd.push, sd.delta, c->stream));
+    rc = run_drive(c, s, st, sch, sd, false, false, stats);
+    if (rc) { (void)ctx_sync(c); return rc; }
+    if (g->rl.active) HIPCHK(launch_unpermute(dist, g->rl.old2new, g->n, d_dist, c->stream));
+    return ctx_sync(c);
 }
 
 int np_bfs(np_ctx *c, np_graph *g, int32_t src, const np_sched *sched, int32_t *level_out, np_stats *stats) {
