@@ -1640,8 +1640,12 @@ int np_batch_create(np_ctx *c, int32_t count, np_graph *const *graphs, const int
     for (size_t j = 0; j < uniq.size() && rc == NP_OK; ++j) {
         const std::vector<int> &ids = by_graph[j];
         const int k = (int)ids.size();
-        if (!c->pack || k < 2) { rest.insert(rest.end(), ids.begin(), ids.end()); continue; }
-        const int npk = (k + PACK_MAX - 1) / PACK_MAX;  // packs of near-equal size
+        // a pack's distance rows are indexed with 32-bit offsets: n x (sources, rounded up to a
+        // power of two) stays within 2^31
+        int cap = PACK_MAX;
+        while (cap > 1 && (long long)uniq[j]->n * cap > (1ll << 31)) cap >>= 1;
+        if (!c->pack || k < 2 || cap < 2) { rest.insert(rest.end(), ids.begin(), ids.end()); continue; }
+        const int npk = (k + cap - 1) / cap;  // packs of near-equal size
         for (int p = 0, at = 0; p < npk && rc == NP_OK; ++p) {
             const int sz = k / npk + (p < k % npk ? 1 : 0);
             rc = build_pack(c, b, uniq[j], std::vector<int>(ids.begin() + at, ids.begin() + at + sz), srcs);

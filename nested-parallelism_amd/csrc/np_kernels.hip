@@ -980,153 +980,151 @@ __device__ __forceinline__ int nth_set_bit(unsigned m, int k) {
     return pos;
 }
 
-// Lane-bin vertices of a packed expansion have out-degree < 2^(PACK_LANE_CLS - 1), so a
-// 64-vertex tile holds < 2^27 items (int) and item / P is exact in float arithmetic.
-constexpr int PACK_LANE_CLS = 17;
+// Lane-bin vertices of a packed expansion have out-degree < 2^(PACK_LANE_CLS - 1): each lane
+// of a lane tile walks the edges of one vertex in sequence (pk_lane_tile), so the bin is kept to
+// short rows; longer ones go to the wavefront and wider bins whatever the schedule says.
+constexpr int PACK_LANE_CLS = 10;
 
-// The relaxation of one item: atomicMin, then the improved source's bit goes to mflag[v] and
-// the first bit of mflag[v] sets v's frontier bit.
-__device__ __forceinline__ void pk_relax_tail(const DevState &st, int v, int src, int alt, int *p) {
-    const int old = atomicMin(p, alt);
-    if (alt >= old || st.cls[v] == 0) return;  // no improvement, or nothing to propagate
-    const unsigned prev = atomicOr(&st.mflag[v], 1u << src);
-    if (prev == 0) atomicOr(&st.bm[0][v >> 5], 1u << (v & 31));
-}
-
-// The relaxation of one item per lane (every lane of the wave calls it together): atomicMin
-// where alt improves on the loaded value, then the improved source bits go to mflag[v].  The
-// lanes of one edge are adjacent (items are edge-major), so several of them can improve the
-// same vertex at once; same-address atomics of one instruction are serialised, so a run of
-// adjacent lanes with the same target ORs its bits first and only the run's last lane updates.
-__device__ __forceinline__ void pk_relax_item(const DevState &st, bool live, int v, int src, int alt,
-                                              int cur, int ks) {
-    unsigned bit = 0;
-    if (live && alt < cur) {
-        const int old = atomicMin(st.dist + ((long long)v << ks) + src, alt);
-        if (alt < old && st.cls[v] != 0) bit = 1u << src;  // class 0: nothing to propagate
+// UNR packed relaxation steps of a wavefront: in step k every lane holds one (edge, source)
+// item -- target tv[k], pending source s, candidate alt[k] = dist[u][s] + w, loaded dist[v][s]
+// cur[k] -- or none (ok[k] false).  The lanes of one edge form a run of consecutive lanes (rs =
+// the run's first lane inside this wavefront, last = this lane ends its run), so they share v.
+// The reference's relaxation per item (sssp_kernel.cu:46-52): if (dist[v][s] > alt) atomicMin
+// (the UNR atomics are issued before any result is used); each improved source's bit then goes
+// to mflag[v], OR-ed across the run first so that one lane per run updates it, and the first bit
+// of mflag[v] sets v's frontier bit.  Vertices without out-edges are flagged too and dropped by
+// the compaction.  Every lane of the wavefront calls it together.
+__device__ __forceinline__ void pk_relax_steps(const DevState &st, const bool *ok, const int *tv, int s,
+                                               const int *alt, const int *cur, int rs, bool last) {
+    int old[UNR];
+#pragma unroll
+    for (int k = 0; k < UNR; ++k) {
+        old[k] = 0x7FFFFFFF;
+        if (ok[k] && alt[k] < cur[k])
+            old[k] = atomicMin(st.dist + (((unsigned)tv[k] << st.kshift) + (unsigned)s), alt[k]);
     }
-    if (__ballot(bit != 0) == 0) return;
     const int lane = lane_id();
-    // every lane takes part in the shuffle: a lane left out returns 0 to the lane reading it
-    const int vprev = __shfl_up(v, 1);
-    const bool head = lane == 0 || vprev != v;
-    if (__ballot(bit != 0 && !head) != 0) {
-        const unsigned long long heads = __ballot(head);
-        const int start = 63 - __clzll(heads & ((2ull << lane) - 1));  // lane 63: all ones
+#pragma unroll
+    for (int k = 0; k < UNR; ++k) {
+        const unsigned bit = (ok[k] && alt[k] < cur[k] && alt[k] < old[k]) ? 1u << s : 0u;
+        if (__ballot(bit != 0) == 0) continue;
         unsigned acc = bit;
 #pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
+        for (int d = 1; d < 32; d <<= 1) {  // runs are at most 32 lanes long (one per source)
             const unsigned y = __shfl_up(acc, d);
-            if (lane - d >= start) acc |= y;
+            if (lane - d >= rs) acc |= y;
         }
-        bit = (lane == 63 || ((heads >> (lane + 1)) & 1ull)) ? acc : 0u;
-    }
-    if (bit) {
-        const unsigned prev = atomicOr(&st.mflag[v], bit);
-        if (prev == 0) atomicOr(&st.bm[0][v >> 5], 1u << (v & 31));
+        if (last && ok[k] && acc) {
+            const int v = tv[k];
+            const unsigned prev = atomicOr(&st.mflag[v], acc);
+            if (prev == 0) atomicOr(&st.bm[0][v >> 5], 1u << (v & 31));
+        }
     }
 }
 
-// Items of one queued vertex u, edges [e0, e1) x pending sources M; worker t of `stride`.
-template <bool UNIT, bool GUARD>
-__device__ __forceinline__ void pk_relax_vertex(const DevState &st, int u, unsigned M, int e0, int e1,
-                                                int t, int stride) {
+// Items of one queued vertex u -- edges [e0, e1) x the P pending sources of M -- on `stride`
+// workers (t = this worker's index; M, e0, e1 are the same for all of them).  Worker t takes
+// source j = t mod P, whose distance at u it reads once, and the edges e0 + t / P + k q,
+// q = stride / P: the P workers of an edge are adjacent and share its col/w read and the cache
+// line of dist[v][*]; the last stride mod P workers idle.
+template <bool UNIT>
+__device__ __forceinline__ void pk_relax_vertex(const DevState &st, int u, unsigned M, int e0, int e1, int t,
+                                                int stride) {
+    M = (unsigned)__builtin_amdgcn_readfirstlane((int)M);
     const int P = __popc(M);
     if (P == 0 || e0 >= e1) return;
     const int ks = st.kshift;
-    const int qs = stride / P, rs = stride - qs * P;  // one step of `stride` items
-    int e = e0 + t / P, j = t - (t / P) * P;
-    const int *du_row = st.dist + ((long long)u << ks);
-    // The persistent kernel keeps every lane to the end (pk_relax_item is wave-wide); the
-    // per-iteration kernels let each lane leave and relax lane by lane (measured faster there).
-    while (GUARD ? e < e1 : __ballot(e < e1) != 0) {
-        int tv[UNR], alt[UNR], cur[UNR], src[UNR];
+    const int q = stride / P;
+    const int eo = t / P, j = t - eo * P;
+    const bool on = eo < q;
+    const int s = on ? nth_set_bit(M, j) : 0;
+    const int du = on ? ld_coherent(st.dist + (((unsigned)u << ks) + (unsigned)s)) : 0;
+    const int lane = lane_id();
+    const int rs = lane - j > 0 ? lane - j : 0;
+    const bool last = j == P - 1 || lane == 63;
+    for (int e = e0 + eo; __ballot(on && e < e1) != 0; e += UNR * q) {
+        int tv[UNR], alt[UNR], cur[UNR];
         bool ok[UNR];
 #pragma unroll
         for (int k = 0; k < UNR; ++k) {
-            ok[k] = e < e1;
-            const int ec = ok[k] ? e : e0;
-            src[k] = nth_set_bit(M, j);
+            const int ek = e + k * q;
+            ok[k] = on && ek < e1;
+            const unsigned ec = ok[k] ? ek : e0;
             tv[k] = st.col[ec];
-            const int du = ld_coherent(du_row + src[k]);  // in u's row for every j < P
-            alt[k] = (GUARD && !ok[k]) ? 0 : du + (UNIT ? 1 : st.w[ec]);
-            e += qs; j += rs;
-            if (j >= P) { j -= P; e += 1; }
+            alt[k] = du + (UNIT ? 1 : st.w[ec]);
         }
 #pragma unroll
-        for (int k = 0; k < UNR; ++k) cur[k] = (GUARD && !ok[k]) ? 0 : ld_coherent(st.dist + ((long long)tv[k] << ks) + src[k]);
-#pragma unroll
-        for (int k = 0; k < UNR; ++k) {
-            if (GUARD) {
-                if (ok[k] && alt[k] < cur[k])
-                    pk_relax_tail(st, tv[k], src[k], alt[k], st.dist + ((long long)tv[k] << ks) + src[k]);
-            } else {
-                pk_relax_item(st, ok[k], tv[k], src[k], alt[k], cur[k], ks);
-            }
-        }
+        for (int k = 0; k < UNR; ++k)
+            cur[k] = ok[k] ? ld_coherent(st.dist + (((unsigned)tv[k] << ks) + (unsigned)s)) : 0;
+        pk_relax_steps(st, ok, tv, s, alt, cur, rs, last);
     }
 }
 
-// Lane bin: 64 queued vertices, their items balanced over the lanes.
-template <bool UNIT, bool GUARD>
+// Lane bin: 64 queued vertices (queue[qidx .. qidx+nvalid)) of one out-degree class.  Their
+// (vertex, pending source) pairs go to the lanes, 64 at a time, the pairs of one vertex adjacent;
+// each lane walks its vertex's edges in order for its source, so the lanes of one vertex read the
+// same edge (one col/w address, one dist[v][*] line) at every step.
+template <bool UNIT>
 __device__ __forceinline__ void pk_lane_tile(const DevState &st, const int *q, const unsigned *qm,
                                              long long qidx, int nvalid) {
     const int lane = lane_id();
     const int ks = st.kshift;
-    int u = 0, s = 0, P = 0, items = 0;
+    int u = 0, s0 = 0, d = 0, P = 0;
     unsigned M = 0;
     if (lane < nvalid) {
         u = ld_coherent(q + qidx + lane);
         M = ld_coherent_u(qm + qidx + lane);
-        s = st.row[u];
-        P = __popc(M);
-        items = (st.row[u + 1] - s) * P;
+        s0 = st.row[u];
+        d = st.row[u + 1] - s0;
+        P = d > 0 ? __popc(M) : 0;
     }
-    const float rcp = P ? 1.0f / (float)P : 0.0f;
-    const int incl = wave_incl_scan(items);
+    const int incl = wave_incl_scan(P);
     const int total = __shfl(incl, 63);
-    const int excl = incl - items;
+    const int excl = incl - P;
     if (total == 0) return;
-    const int first_lane = __ffsll((long long)__ballot(items > 0)) - 1;
-    const int e_safe = __shfl(s, first_lane);
-    for (int base = 0; base < total; base += UNR * 64) {
-        int tv[UNR], alt[UNR], cur[UNR], src[UNR];
-        bool ok[UNR];
+    // lanes past the last pair read the first edge of the first vertex with pairs (a valid edge)
+    const int e_safe = __shfl(s0, __ffsll((long long)__ballot(P > 0)) - 1);
+    for (int base = 0; base < total; base += 64) {
+        const int k = base + lane;
+        const bool on = k < total;
+        int o = 0;  // owner: the last lane whose exclusive offset is <= k
 #pragma unroll
-        for (int k = 0; k < UNR; ++k) {
-            const int kk = base + lane + k * 64;
-            ok[k] = kk < total;
-            int o = 0;  // owner: the last lane whose exclusive offset is <= kk
-#pragma unroll
-            for (int step = 32; step > 0; step >>= 1) {
-                int x = __shfl(excl, o + step);
-                if (x <= kk) o += step;
-            }
-            const int r = kk - __shfl(excl, o);
-            const int Po = __shfl(P, o);
-            const float ro = __shfl(rcp, o);
-            const unsigned Mo = __shfl(M, o);
-            const int so = __shfl(s, o);
-            const int uo = __shfl(u, o);
-            int eo = (int)((float)r * ro);  // r / Po, then corrected by at most one
-            int jo = r - eo * Po;
-            if (jo < 0) { eo -= 1; jo += Po; }
-            else if (jo >= Po) { eo += 1; jo -= Po; }
-            // lanes past the end read the tile's first edge from vertex uo's row, source 0
-            const int ec = ok[k] ? so + eo : e_safe;
-            src[k] = ok[k] ? nth_set_bit(Mo, jo) : 0;
-            tv[k] = st.col[ec];
-            const int du = ld_coherent(st.dist + ((long long)uo << ks) + src[k]);  // uo is a vertex
-            alt[k] = (GUARD && !ok[k]) ? 0 : du + (UNIT ? 1 : st.w[ec]);
+        for (int step = 32; step > 0; step >>= 1) {
+            const int x = __shfl(excl, o + step);
+            if (x <= k) o += step;
         }
+        const int j = k - __shfl(excl, o);
+        const int Po = __shfl(P, o);
+        const unsigned Mo = __shfl(M, o);
+        const int uo = __shfl(u, o);
+        // every lane takes part in each shuffle: a lane left out would return 0 to its readers
+        const int so = __shfl(s0, o);
+        const int dso = __shfl(d, o);
+        const int e0 = on ? so : e_safe;
+        const int deg = on ? dso : 0;
+        const int s = on ? nth_set_bit(Mo, j) : 0;
+        const int du = on ? ld_coherent(st.dist + (((unsigned)uo << ks) + (unsigned)s)) : 0;
+        const int rs = lane - j > 0 ? lane - j : 0;
+        const bool last = j == Po - 1 || lane == 63;
+        for (int i = 0; __ballot(i < deg) != 0; i += UNR) {
+            int tv[UNR], alt[UNR], cur[UNR];
+            bool ok[UNR];
 #pragma unroll
-        for (int k = 0; k < UNR; ++k) cur[k] = (GUARD && !ok[k]) ? 0 : ld_coherent(st.dist + ((long long)tv[k] << ks) + src[k]);
+            for (int kk = 0; kk < UNR; ++kk) {
+                ok[kk] = i + kk < deg;
+                const unsigned ec = e0 + (ok[kk] ? i + kk : 0);  // a lane with deg 0 reads a valid edge of its tile
+                tv[kk] = st.col[ec];
+                alt[kk] = du + (UNIT ? 1 : st.w[ec]);
+            }
 #pragma unroll
-        for (int k = 0; k < UNR; ++k) pk_relax_item(st, ok[k], tv[k], src[k], alt[k], cur[k], ks);
+            for (int kk = 0; kk < UNR; ++kk)
+                cur[kk] = ok[kk] ? ld_coherent(st.dist + (((unsigned)tv[kk] << ks) + (unsigned)s)) : 0;
+            pk_relax_steps(st, ok, tv, s, alt, cur, rs, last);
+        }
     }
 }
 
-template <bool UNIT, bool GUARD>
+template <bool UNIT>
 __device__ void pk_expansion(const DevState &st, const int *q, const unsigned *qm, ItemTables &tb) {
     const long long G = tb.pre[0][NCLS], K = tb.pre[1][NCLS], W = tb.pre[2][NCLS],
                     T = tb.pre[3][NCLS];
@@ -1135,7 +1133,7 @@ __device__ void pk_expansion(const DevState &st, const int *q, const unsigned *q
         long long qi = tb.seg[c] + (k - tb.pre[1][c]);
         int u = ld_coherent(q + qi);
         unsigned M = ld_coherent_u(qm + qi);
-        pk_relax_vertex<UNIT, GUARD>(st, u, M, st.row[u], st.row[u + 1], threadIdx.x, WG);
+        pk_relax_vertex<UNIT>(st, u, M, st.row[u], st.row[u + 1], threadIdx.x, WG);
     }
     const long long NW = (long long)gridDim.x * WAVES;
     const long long gw = (long long)blockIdx.x * WAVES + (threadIdx.x >> 6);
@@ -1153,21 +1151,21 @@ __device__ void pk_expansion(const DevState &st, const int *q, const unsigned *q
             int e0 = s + (j << GRID_CHUNK_SHIFT);
             int e1 = e0 + (1 << GRID_CHUNK_SHIFT);
             if (e1 > e) e1 = e;
-            pk_relax_vertex<UNIT, GUARD>(st, u, M, e0, e1, lane_id(), 64);
+            pk_relax_vertex<UNIT>(st, u, M, e0, e1, lane_id(), 64);
         } else if (x < G + W) {
             long long i = x - G;
             int c = find_class(tb.pre[2], i);
             long long qi = tb.seg[c] + (i - tb.pre[2][c]);
             int u = ld_coherent(q + qi);
             unsigned M = ld_coherent_u(qm + qi);
-            pk_relax_vertex<UNIT, GUARD>(st, u, M, st.row[u], st.row[u + 1], lane_id(), 64);
+            pk_relax_vertex<UNIT>(st, u, M, st.row[u], st.row[u + 1], lane_id(), 64);
         } else {
             long long t = x - G - W;
             int c = find_class(tb.pre[3], t);
             long long ti = t - tb.pre[3][c];
             long long first = ti * 64;
             int nvalid = (int)((long long)tb.cnt[c] - first < 64 ? (long long)tb.cnt[c] - first : 64);
-            pk_lane_tile<UNIT, GUARD>(st, q, qm, tb.seg[c] + first, nvalid);
+            pk_lane_tile<UNIT>(st, q, qm, tb.seg[c] + first, nvalid);
         }
     }
 }
@@ -1233,7 +1231,7 @@ __device__ __forceinline__ void pk_compaction(const DevState &st, int *cnt_out, 
         for (unsigned w = word; w; w &= w - 1) {
             const int b = __ffs(w) - 1;
             const int c = st.cls[wi * 32 + b];
-            if (!c) continue;  // class 0 is never flagged; dropped regardless
+            if (!c) continue;  // no out-edges: dropped (its mflag stays set, so it is not flagged again)
             queued |= 1u << b;
             atomicAdd(&s_lc[c], 1);
         }
@@ -1272,7 +1270,7 @@ __device__ __forceinline__ void pk_compaction(const DevState &st, int *cnt_out, 
             qt[k] = 0;
             if ((word >> (v & 31)) & 1u) {
                 const int c = st.cls[v];
-                if (c) {  // class 0 is never flagged; dropped regardless
+                if (c) {  // no out-edges: dropped (its mflag stays set, so it is not flagged again)
                     const unsigned m = ld_coherent_u(st.mflag + v);
                     int dmin;
                     const unsigned t = pk_take(st, v, m, T, &dmin);
@@ -1395,7 +1393,7 @@ __global__ __launch_bounds__(WG) void k_pk_persistent(DevState st, SchedDev sp) 
             record_trace(sp, it, cfg, tb.cnt, t_prev, ctrl);
         }
         __syncthreads();
-        pk_expansion<UNIT, false>(st, q_cur, qm_cur, tb);
+        pk_expansion<UNIT>(st, q_cur, qm_cur, tb);
         if (!grid_sync(ctrl, nb, gen)) return;
     }
 }
@@ -1432,7 +1430,7 @@ __global__ __launch_bounds__(WG) void k_pk_expand(DevState st, int it_off) {
         build_tables(tb, pk_cfg(cfg));
     }
     __syncthreads();
-    pk_expansion<UNIT, true>(st, st.queue[it & 1], st.qmask[it & 1], tb);
+    pk_expansion<UNIT>(st, st.queue[it & 1], st.qmask[it & 1], tb);
 }
 
 __global__ void k_pk_init_sources(DevState st, const int *srcs, int nsrc, int T0) {
