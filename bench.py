@@ -12,6 +12,10 @@ distances verified identical):
   concat   all instances in one persistent launch over a concatenated slot space
   batched  np_batch with source packing: the sources of one graph share distance rows
            (one persistent launch per 32 sources), bins from the batch autotuner
+  learned_batched  the same batch with the trained MLP picking level and bins per iteration,
+           on the batch autotuner's drive and window (its fixed bins are the fallback)
+The learned modes run on the drive and priority window their tuned counterparts chose, so
+learned vs tuned compares the per-iteration choice of bins only.
 The JSON value is the fastest mode; every mode is listed under "modes".
 The other configurations (--config c1|c2|c4|c5) are the parity/evaluation cases of the
 same path (ER 64K, R-MAT s22 per level, many sources + many uneven graphs, grid road
@@ -500,8 +504,8 @@ def run_config(npa, ctx, args):
     dbuf = npa.DeviceBuffer(ctx, nmax)
 
     default_modes = {"c1": "plain,tuned", "c2": "level0,level1,level2,tuned,batched",
-                     "c3": "plain,tuned,learned,concat,batched",
-                     "c4": "plain,tuned,concat,batched", "c5": "plain,tuned,learned,batched"}[args.config]
+                     "c3": "plain,tuned,learned,concat,batched,learned_batched",
+                     "c4": "plain,tuned,concat,batched", "c5": "plain,tuned,learned,batched,learned_batched"}[args.config]
     modes = [m for m in (args.modes or default_modes).split(",") if m]
     scheds = {"plain": plain, "level0": npa.Sched(level=0), "level1": npa.Sched(level=1),
               "level2": npa.Sched(level=2)}
@@ -513,7 +517,7 @@ def run_config(npa, ctx, args):
                                 drive=v[4] if len(v) > 4 else npa.DRIVE_PERSISTENT, delta=v[5] if len(v) > 5 else 0)
     if "tuned" in fixed:
         scheds["tuned"] = fixed["tuned"]
-    elif "tuned" in modes or "concat" in modes:
+    elif "tuned" in modes or "concat" in modes or "learned" in modes:
         t = time.time()
         big = max(range(len(W.graphs)), key=lambda i: W.graphs[i].m)
         srcs = [s for gi, s in W.inst if gi == big][:2]
@@ -523,29 +527,32 @@ def run_config(npa, ctx, args):
     model_path = os.path.join(PKG_DIR, "sched_mlp.f32")
     if os.path.exists(model_path):
         ctx.model_set(np.fromfile(model_path, dtype=np.float32))
-        scheds["learned"] = npa.Sched(level=npa.LEVEL_LEARNED, log2_t_lo=6, log2_t_hi=6)
-    elif "learned" in modes:
-        log("no trained scheduler model at %s: learned mode skipped" % model_path)
-        modes = [m for m in modes if m != "learned"]
+        if "tuned" in scheds:
+            scheds["learned"] = scheds["tuned"].copy(level=npa.LEVEL_LEARNED)
+    elif "learned" in modes or "learned_batched" in modes:
+        log("no trained scheduler model at %s: learned modes skipped" % model_path)
+        modes = [m for m in modes if m not in ("learned", "learned_batched")]
 
     results, finals, batches = {}, {}, {}
     for mode in modes:
-        if mode in ("batched", "concat"):
+        if mode in ("batched", "concat", "learned_batched"):
             # batched: instances sharing a graph source-packed; concat: every instance in the
             # concatenated slot space (packing off)
-            ctx.set_option(npa.CTX_PACK, 1 if mode == "batched" else 0)
+            ctx.set_option(npa.CTX_PACK, 0 if mode == "concat" else 1)
             B = npa.Batch(ctx, [W.graphs[gi] for gi, _ in W.inst], [s for _, s in W.inst])
             ctx.set_option(npa.CTX_PACK, 1)
             batches[mode] = B
-            if mode == "batched" and "batched" in fixed:
-                scheds[mode] = fixed["batched"]
-            elif mode == "batched":
+            if mode == "concat":
+                scheds[mode] = scheds["tuned"]
+            elif "batched" in fixed:
+                scheds["batched"] = fixed["batched"]
+            elif "batched" not in scheds:
                 t = time.time()
                 bs, bms = B.autotune(npa.Sched(level=1))
-                scheds[mode] = bs.copy(trace=0)
+                scheds["batched"] = bs.copy(trace=0)
                 log("batch autotune: %r (%.3f ms/solve, %.1fs); layout %s" % (bs, bms, time.time() - t, B.layout()))
-            else:
-                scheds[mode] = scheds["tuned"]
+            if mode == "learned_batched":
+                scheds[mode] = scheds["batched"].copy(level=npa.LEVEL_LEARNED)
 
             def step(B=B, sch=scheds[mode]):
                 st = B.solve(sch)
@@ -655,6 +662,10 @@ def run_config(npa, ctx, args):
                   for k, v in results.items()},
         "speedup_learned_vs_plain": round(results["learned"]["gteps"] / base, 3) if base and "learned" in results else None,
         "speedup_batched_vs_plain": round(results["batched"]["gteps"] / base, 3) if base and "batched" in results else None,
+        "speedup_learned_batched_vs_plain": (round(results["learned_batched"]["gteps"] / base, 3)
+                                             if base and "learned_batched" in results else None),
+        "speedup_learned_vs_tuned": (round(results["learned"]["gteps"] / results["tuned"]["gteps"], 3)
+                                     if "learned" in results and "tuned" in results else None),
     }
     print(json.dumps(line), flush=True)
     if args.copy_calibration > 0:
