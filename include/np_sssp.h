@@ -143,10 +143,11 @@ typedef struct np_stats {
     int32_t pad;
     int64_t pairs_total;    /* sum over iterations of queued (vertex, source) pairs: equals
                                frontier_total except in source-packed batch parts */
-    int64_t edges_total;    /* source-packed parts: sum over iterations of the out-degrees of
-                               the queued vertices (edge rows read); 0 elsewhere */
-    int64_t items_total;    /* source-packed parts: sum of out-degree x queued sources
-                               ((edge, source) relaxations attempted); 0 elsewhere */
+    int64_t edges_total;    /* source-packed parts of traced solves (sched.trace) on the
+                               host-loop and graph drives: sum over iterations of the
+                               out-degrees of the queued vertices (edge rows read); 0 elsewhere */
+    int64_t items_total;    /* same solves: sum of out-degree x queued sources ((edge, source)
+                               relaxations attempted); 0 elsewhere */
     int64_t reserved[2];
 } np_stats;
 
@@ -284,9 +285,9 @@ typedef struct np_trace_rec {
     int32_t log2_t_lo, log2_t_hi;
     int32_t count[NP_NCLASS]; /* frontier size per out-degree class */
     int64_t t_begin, t_end;   /* device clock (100 MHz) at the start/end of the expansion */
-    int64_t pairs, edges;     /* source-packed parts: queued (vertex, source) pairs and the
-                                 queued vertices' out-degrees, summed over iterations up to
-                                 and including this one; 0 elsewhere */
+    int64_t pairs, edges;     /* source-packed parts: queued (vertex, source) pairs and (host-loop
+                                 and graph drives) the queued vertices' out-degrees, summed over
+                                 iterations up to and including this one; 0 elsewhere */
 } np_trace_rec;
 /* Copies up to `cap` records of the last solve on ctx; *count = records available. */
 int np_trace_get(np_ctx *ctx, np_trace_rec *recs, int32_t cap, int32_t *count);

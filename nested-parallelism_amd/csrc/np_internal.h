@@ -59,8 +59,8 @@ struct DevState {
     // counters [3], rotating by iteration
     int *sbuf;
     int *sbcnt;
-    // source-packed: accumulate Ctrl::edges_total / items_total in the compaction (one row
-    // read per queued vertex; set when the solve records a trace)
+    // source-packed, host-loop and graph drives: a counting kernel after each compaction
+    // accumulates Ctrl::edges_total / items_total (set when the solve records a trace)
     int count_work;
 };
 

@@ -1219,7 +1219,6 @@ __device__ __forceinline__ void pk_compaction(const DevState &st, int *cnt_out, 
     if (threadIdx.x < NCLS) s_lc[threadIdx.x] = 0;
     __syncthreads();
     int pairs = 0, my_dc = 0, my_dm = NO_WINDOW;
-    long long edges = 0, items = 0;
     const long long nv = st.nwords * 32;
     // Without a window every pending vertex is taken whole and nothing stays flagged: the frontier
     // is as sparse as a single source's, so a thread scans one bitmap word as compaction_impl does.
@@ -1245,11 +1244,6 @@ __device__ __forceinline__ void pk_compaction(const DevState &st, int *cnt_out, 
             st_coherent(q_out + qi, (int)v);
             st_coherent_u(qm_out + qi, t);
             pairs += __popc(t);
-            if (st.count_work) {
-                const long long d = st.row[v + 1] - st.row[v];
-                edges += d;
-                items += d * __popc(t);
-            }
         }
         if (word) st_coherent_u(bm + wi, 0u);
         __syncthreads();
@@ -1298,27 +1292,12 @@ __device__ __forceinline__ void pk_compaction(const DevState &st, int *cnt_out, 
             st_coherent(q_out + qi, (int)v);
             st_coherent_u(qm_out + qi, qt[k]);
             pairs += __popc(qt[k]);
-            if (st.count_work) {
-                const long long d = st.row[v + 1] - st.row[v];
-                edges += d;
-                items += d * __popc(qt[k]);
-            }
         }
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) pairs += __shfl_xor(pairs, o);
     if (lane_id() == 0 && pairs) atomicAdd((unsigned long long *)&st.ctrl->pairs_total, (unsigned long long)pairs);
-    if (st.count_work) {
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            edges += __shfl_xor(edges, o);
-            items += __shfl_xor(items, o);
-        }
-        if (lane_id() == 0 && edges) {
-            atomicAdd((unsigned long long *)&st.ctrl->edges_total, (unsigned long long)edges);
-            atomicAdd((unsigned long long *)&st.ctrl->items_total, (unsigned long long)items);
-        }
-    }
+
     if (T != NO_WINDOW) flush_deferred(my_dc, my_dm, &s_dc, &s_dm, defer_cnt, defer_min);
 }
 
@@ -1413,8 +1392,53 @@ __global__ __launch_bounds__(WG) void k_pk_compact(DevState st, int it_off) {
                   ctrl->defer_cnt + pass % 3, ctrl->defer_min + pass % 3);
 }
 
+// The packed expansion is a latency-bound gather: it is held to 64 VGPRs so that eight waves share
+// each SIMD (C3 packed, graph drive: 41.7 -> 31.3 ms per solve, a few registers spilled).  The
+// persistent kernel keeps the compiler's allocation: held to 64 it spills its compaction and
+// window code, which made C3 faster (42.2 -> 37.2 ms) but the grid with a window slower
+// (125 -> 155 ms), and the graph drive is the faster drive for both.
+// Work of the queue the last compaction built (traced solves on the host-loop and graph drives):
+// out-degrees of the queued vertices into Ctrl::edges_total, out-degree x queued sources into
+// Ctrl::items_total.  Kept out of the compaction, where even the untaken counting branch cost
+// the windowed grid 20 %.
+__global__ __launch_bounds__(WG) void k_pk_count(DevState st, int it_off) {
+    __shared__ long long s_pre[NCLS + 1];
+    __shared__ int s_seg[NCLS];
+    Ctrl *ctrl = st.ctrl;
+    if (ld_coherent(&ctrl->done) || ld_coherent(&ctrl->error)) return;
+    const int it = ld_coherent(&ctrl->it_base) + it_off;
+    if (threadIdx.x < NCLS) s_seg[threadIdx.x] = st.seg_off[threadIdx.x];
+    if (threadIdx.x == 0) {
+        int lc[NCLS];
+        for (int c = 0; c < NCLS; ++c) lc[c] = ld_coherent(st.cnt + (it % 3) * NCLS + c);
+        sol_prefix(lc, 1, NCLS, s_pre);
+    }
+    __syncthreads();
+    const long long total = s_pre[NCLS];
+    const int *q = st.queue[it & 1];
+    const unsigned *qm = st.qmask[it & 1];
+    unsigned long long edges = 0, items = 0;
+    for (long long i = (long long)blockIdx.x * WG + threadIdx.x; i < total; i += (long long)gridDim.x * WG) {
+        const int c = find_class(s_pre, i);
+        const long long qi = s_seg[c] + (i - s_pre[c]);
+        const int u = ld_coherent(q + qi);
+        const unsigned d = (unsigned)(st.row[u + 1] - st.row[u]);
+        edges += d;
+        items += (unsigned long long)d * __popc(ld_coherent_u(qm + qi));
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        edges += __shfl_xor(edges, o);
+        items += __shfl_xor(items, o);
+    }
+    if (lane_id() == 0 && edges) {
+        atomicAdd((unsigned long long *)&ctrl->edges_total, edges);
+        atomicAdd((unsigned long long *)&ctrl->items_total, items);
+    }
+}
+
 template <bool UNIT>
-__global__ __launch_bounds__(WG) void k_pk_expand(DevState st, int it_off) {
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8))) void k_pk_expand(DevState st, int it_off) {
     __shared__ ItemTables tb;
     Ctrl *ctrl = st.ctrl;
     if (ld_coherent(&ctrl->done) || ld_coherent(&ctrl->error) || ld_coherent(&ctrl->skip)) return;
@@ -1852,8 +1876,12 @@ hipError_t launch_compact(const DevState &st, int it_off, bool batch, int wgs, h
     (void)batch;
     long long need = (st.nwords + WG - 1) / WG;
     int g = (int)(need < wgs ? (need < 1 ? 1 : need) : wgs);
-    if (st.kshift >= 0) hipLaunchKernelGGL(k_pk_compact, dim3(g), dim3(WG), 0, s, st, it_off);
-    else hipLaunchKernelGGL(k_compact, dim3(g), dim3(WG), 0, s, st, it_off);
+    if (st.kshift >= 0) {
+        hipLaunchKernelGGL(k_pk_compact, dim3(g), dim3(WG), 0, s, st, it_off);
+        if (st.count_work) hipLaunchKernelGGL(k_pk_count, dim3(wgs), dim3(WG), 0, s, st, it_off);
+    } else {
+        hipLaunchKernelGGL(k_compact, dim3(g), dim3(WG), 0, s, st, it_off);
+    }
     return hipGetLastError();
 }
 hipError_t launch_init_packed(const DevState &st, const int *d_srcs, int nsrc, int delta, hipStream_t s) {
