@@ -21,7 +21,7 @@ backward kernels).  Output: sched_mlp.f32 (NP_MLP_NPARAM float32) next to this f
 a JSON summary (dataset size, regret of the plain schedule and of the network on held-out
 graphs -- the last 20% of each family -- per family).
 
-    python nested-parallelism_amd/train_scheduler.py [--rmat-runs 120] [--er-runs 30] [--grid-runs 30]
+    python nested-parallelism_amd/train_scheduler.py [--rmat-runs 200] [--er-runs 30] [--grid-runs 30]
 """
 import argparse
 import importlib.util
@@ -136,7 +136,7 @@ def predict_index(out, cands):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--rmat-runs", type=int, default=120)
+    ap.add_argument("--rmat-runs", type=int, default=200)
     ap.add_argument("--er-runs", type=int, default=30)
     ap.add_argument("--grid-runs", type=int, default=30)
     ap.add_argument("--scales", default="18,19,20,21,22")
