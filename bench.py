@@ -625,7 +625,7 @@ def run_config(npa, ctx, args):
     if os.path.exists(tpath):
         try:
             tj = json.load(open(tpath))
-            rec = tj.get(head)
+            rec = tj.get(head) or (tj.get("batched") if head == "learned_batched" else None)
             if isinstance(rec, dict):
                 traffic = rec.get("bytes")
                 traffic_note = {"source": "stored, not measured in this run: profiles/traffic_%s.json" % args.config,
@@ -633,6 +633,19 @@ def run_config(npa, ctx, args):
                                 "write_scale": rec.get("write_scale"), "profile": rec.get("profile")}
         except (ValueError, OSError):
             traffic = None
+    # work of one traced solve of the fastest mode (not timed): edge rows read (out-degrees of the queued
+    # vertices), (edge, source) items and queued (vertex, source) pairs; for packs the mean number of
+    # sources pending per edge row read and the re-expansion of pairs over the reached ones
+    work = None
+    if head in batches:
+        st = batches[head].solve(scheds[head].copy(trace=1))
+        n_r_total = int(n_r.sum())
+        work = {"queued_pairs": int(st.pairs_total), "queued_pairs_over_reached": round(st.pairs_total / max(1, n_r_total), 3),
+                "iterations": int(st.iterations)}
+        if st.edges_total:
+            work.update({"edge_rows": int(st.edges_total), "items": int(st.items_total),
+                         "pending_sources_per_edge_row": round(st.items_total / st.edges_total, 2),
+                         "edge_rows_over_m": round(st.edges_total / max(1, sum(g.m for g in W.graphs)), 3)})
     base = results.get("plain", {}).get("gteps")
     line = {
         "metric": METRIC if args.config == "c3" else METRIC + " [config %s]" % args.config,
@@ -657,6 +670,7 @@ def run_config(npa, ctx, args):
                      "kernel": kernel, "bytes_per_launch": per_launch_bytes},
         "cpu_baseline": cpu,
         "exact": exact,
+        "work": work,
         "modes": {k: {"gteps": round(v["gteps"], 3), "ms_per_step": round(v["ms_per_step"], 3),
                       "launch_ms": round(v["launch_ms"], 4), "schedule": repr(scheds.get(k))}
                   for k, v in results.items()},
