@@ -177,6 +177,11 @@ int np_device_info(np_ctx *ctx, int32_t *num_cus, int32_t *max_wgs_per_cu);
 /* NP_CTX_PACK: 1 (default) batch instances that share a graph are solved source-packed
  * (np_batch_create); 0 every instance goes to the concatenated part. */
 #define NP_CTX_PACK 2
+/* NP_CTX_PACK_MAX: the most sources one pack holds, 2..32 (default 32); a graph's k sources
+ * go to ceil(k / max) packs of near-equal size.  Smaller packs read narrower distance rows per
+ * edge (fewer unused bytes when few of a row's sources are pending) and run side by side on the
+ * graph drive. */
+#define NP_CTX_PACK_MAX 3
 int np_set_option(np_ctx *ctx, int32_t key, int32_t value);
 /* 1 when the graph is solved in a renumbered layout, else 0; skew (optional) receives the
  * share of edges that its n/16 most-targeted vertices receive. */

@@ -29,6 +29,7 @@ NCLASS = 32
 MLP_NIN = 10
 CTX_RELABEL = 1           # np_set_option key: 0 caller ids, 1 always renumber, 2 when skewed
 CTX_PACK = 2              # np_set_option key: 1 batch instances sharing a graph are source-packed
+CTX_PACK_MAX = 3          # np_set_option key: most sources per pack (2..32)
 MLP_NPARAM = 32 * 10 + 32 + 32 * 32 + 32 + 5 * 32 + 5
 NOALLOC_MAX_VERTICES = 2097152  # np_sssp_device_noalloc's largest graph
 

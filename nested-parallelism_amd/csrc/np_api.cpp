@@ -140,6 +140,7 @@ struct np_ctx {
     long long snap_n = 0;
     int relabel = 2;  // NP_CTX_RELABEL: 0 never, 1 always, 2 when in-degree is skewed
     int pack = 1;     // NP_CTX_PACK: batch instances sharing a graph go to source-packed states
+    int pack_max = PACK_MAX;  // NP_CTX_PACK_MAX: most sources per pack
     // grow-only device workspace for PageRank and SpMV (their buffers are carved from it, so
     // repeated calls allocate nothing once it is large enough) and a small pinned readback
     char *ws = nullptr;
@@ -756,6 +757,10 @@ int np_set_option(np_ctx *c, int32_t key, int32_t value) {
     case NP_CTX_RELABEL:
         if (value < 0 || value > 2) { set_error("NP_CTX_RELABEL takes 0, 1 or 2"); return NP_ERR_ARG; }
         c->relabel = value;
+        return NP_OK;
+    case NP_CTX_PACK_MAX:
+        if (value < 2 || value > PACK_MAX) { set_error("NP_CTX_PACK_MAX takes 2..%d", PACK_MAX); return NP_ERR_ARG; }
+        c->pack_max = value;
         return NP_OK;
     case NP_CTX_PACK:
         if (value < 0 || value > 1) { set_error("NP_CTX_PACK takes 0 or 1"); return NP_ERR_ARG; }
@@ -1642,7 +1647,7 @@ int np_batch_create(np_ctx *c, int32_t count, np_graph *const *graphs, const int
         const int k = (int)ids.size();
         // a pack's distance rows are indexed with 32-bit offsets: n x (sources, rounded up to a
         // power of two) stays within 2^31
-        int cap = PACK_MAX;
+        int cap = c->pack_max;
         while (cap > 1 && (long long)uniq[j]->n * cap > (1ll << 31)) cap >>= 1;
         if (!c->pack || k < 2 || cap < 2) { rest.insert(rest.end(), ids.begin(), ids.end()); continue; }
         const int npk = (k + cap - 1) / cap;  // packs of near-equal size

@@ -373,6 +373,31 @@ def test_packed_batch(npa, ctx, orc):
         g.free()
 
 
+def test_pack_max_option(npa, ctx, orc):
+    """NP_CTX_PACK_MAX caps the sources per pack: 20 sources on one graph go to ceil(20 / max)
+    packs of near-equal size, each solved exactly; values outside 2..32 are refused."""
+    G = npa.Graph.rmat(ctx, 12, 16, wmin=1, wmax=255, seed=33)
+    h = G.download()
+    srcs = [int(s) for s in npa.pick_sources(h[0], 20, 33)]
+    refs = [orc.dijkstra(*h, s)[0] for s in srcs]
+    for bad in (1, 33):
+        with pytest.raises(npa.NpError):
+            ctx.set_option(npa.CTX_PACK_MAX, bad)
+    try:
+        for pm, parts in ((6, 4), (2, 10), (32, 1)):
+            ctx.set_option(npa.CTX_PACK_MAX, pm)
+            B = npa.Batch(ctx, [G] * len(srcs), srcs)
+            assert B.layout() == (parts, 0), pm
+            for drive in (0, 2):
+                B.solve(_sched(npa, 2, 4, 8, drive, 0, delta=100 if drive else 0))
+                for i, ref in enumerate(refs):
+                    assert np.array_equal(B.get(i), ref), (pm, drive, i)
+            B.free()
+    finally:
+        ctx.set_option(npa.CTX_PACK_MAX, 32)
+    G.free()
+
+
 def test_learned_scheduler_beats_fixed_heuristic(npa, ctx, orc):
     """The trained MLP (sched_mlp.f32, trained on R-MAT s18-22 seeds 100-299) against the
     reference's fixed heuristic (block level, THRESHOLD 64) on a held-out R-MAT s20 graph:
