@@ -1060,6 +1060,69 @@ __device__ __forceinline__ void pk_relax_vertex(const DevState &st, int u, unsig
     }
 }
 
+// Wavefront-scope form of pk_relax_vertex (stride 64) with the neighbour tile staged in LDS: the
+// wavefront copies up to PK_TILE consecutive edges' col (and w) into its own LDS slice with
+// coalesced loads, one edge per lane per load, then takes the edges of its relaxation steps from
+// there.  Without staging every step re-reads col/w from global memory, q = 64 / P distinct edges
+// per load, and waits for them before its distance gathers.  tcol / tw: the wavefront's slice.
+constexpr int PK_TILE = 256;
+constexpr bool PK_STAGE = true;
+// a group of UNR steps spans up to UNR * 64 edges (one pending source): a tile must hold one,
+// or the rounded tile below would be empty and its loop would not advance
+static_assert(PK_TILE >= UNR * 64 && PK_TILE % 64 == 0, "PK_TILE must hold a step group of one source");
+template <bool UNIT>
+__device__ __forceinline__ void pk_relax_vertex_staged(const DevState &st, int u, unsigned M, int e0, int e1,
+                                                       int *tcol, int *tw) {
+    M = (unsigned)__builtin_amdgcn_readfirstlane((int)M);
+    const int P = __popc(M);
+    if (P == 0 || e0 >= e1) return;
+    const int ks = st.kshift;
+    const int lane = lane_id();
+    const int q = 64 / P;
+    const int eo = lane / P, j = lane - eo * P;
+    const bool on = eo < q;
+    const int s = on ? nth_set_bit(M, j) : 0;
+    const int du = on ? ld_coherent(st.dist + (((unsigned)u << ks) + (unsigned)s)) : 0;
+    const int rs = lane - j > 0 ? lane - j : 0;
+    const bool last = j == P - 1 || lane == 63;
+    const int span = UNR * q;                  // edges of one group of UNR steps (<= PK_TILE)
+    const int tile = (PK_TILE / span) * span;  // whole groups per tile: no group straddles two tiles
+    for (int tb = e0; tb < e1; tb += tile) {
+        const int nt = e1 - tb < tile ? e1 - tb : tile;
+        __builtin_amdgcn_wave_barrier();       // the previous tile's reads come first
+        int c[PK_TILE / 64], wv[PK_TILE / 64];
+#pragma unroll
+        for (int r = 0; r < PK_TILE / 64; ++r) {
+            const int i = lane + 64 * r;
+            const int ic = tb + (i < nt ? i : nt - 1);  // lanes past the tile re-read its last edge
+            c[r] = st.col[ic];
+            if (!UNIT) wv[r] = st.w[ic];
+        }
+#pragma unroll
+        for (int r = 0; r < PK_TILE / 64; ++r) {
+            tcol[lane + 64 * r] = c[r];
+            if (!UNIT) tw[lane + 64 * r] = wv[r];
+        }
+        __builtin_amdgcn_wave_barrier();
+        for (int e = eo; __ballot(on && e < nt) != 0; e += span) {
+            int tv[UNR], alt[UNR], cur[UNR];
+            bool ok[UNR];
+#pragma unroll
+            for (int k = 0; k < UNR; ++k) {
+                const int ek = e + k * q;
+                ok[k] = on && ek < nt;
+                const int ec = ok[k] ? ek : 0;
+                tv[k] = tcol[ec];
+                alt[k] = du + (UNIT ? 1 : tw[ec]);
+            }
+#pragma unroll
+            for (int k = 0; k < UNR; ++k)
+                cur[k] = ok[k] ? ld_coherent(st.dist + (((unsigned)tv[k] << ks) + (unsigned)s)) : 0;
+            pk_relax_steps(st, ok, tv, s, alt, cur, rs, last);
+        }
+    }
+}
+
 // Lane bin: 64 queued vertices (queue[qidx .. qidx+nvalid)) of one out-degree class.  Their
 // (vertex, pending source) pairs go to the lanes, 64 at a time, the pairs of one vertex adjacent;
 // each lane walks its vertex's edges in order for its source, so the lanes of one vertex read the
@@ -1138,6 +1201,8 @@ __device__ void pk_expansion(const DevState &st, const int *q, const unsigned *q
     const long long NW = (long long)gridDim.x * WAVES;
     const long long gw = (long long)blockIdx.x * WAVES + (threadIdx.x >> 6);
     const long long U = G + W + T;
+    __shared__ int s_tcol[WAVES][PK_TILE], s_tw[WAVES][PK_TILE];
+    int *tcol = s_tcol[threadIdx.x >> 6], *tw = s_tw[threadIdx.x >> 6];
     for (long long x = gw; x < U; x += NW) {
         if (x < G) {
             int c = find_class(tb.pre[0], x);
@@ -1151,14 +1216,16 @@ __device__ void pk_expansion(const DevState &st, const int *q, const unsigned *q
             int e0 = s + (j << GRID_CHUNK_SHIFT);
             int e1 = e0 + (1 << GRID_CHUNK_SHIFT);
             if (e1 > e) e1 = e;
-            pk_relax_vertex<UNIT>(st, u, M, e0, e1, lane_id(), 64);
+            if (PK_STAGE) pk_relax_vertex_staged<UNIT>(st, u, M, e0, e1, tcol, tw);
+            else pk_relax_vertex<UNIT>(st, u, M, e0, e1, lane_id(), 64);
         } else if (x < G + W) {
             long long i = x - G;
             int c = find_class(tb.pre[2], i);
             long long qi = tb.seg[c] + (i - tb.pre[2][c]);
             int u = ld_coherent(q + qi);
             unsigned M = ld_coherent_u(qm + qi);
-            pk_relax_vertex<UNIT>(st, u, M, st.row[u], st.row[u + 1], lane_id(), 64);
+            if (PK_STAGE) pk_relax_vertex_staged<UNIT>(st, u, M, st.row[u], st.row[u + 1], tcol, tw);
+            else pk_relax_vertex<UNIT>(st, u, M, st.row[u], st.row[u + 1], lane_id(), 64);
         } else {
             long long t = x - G - W;
             int c = find_class(tb.pre[3], t);
