@@ -133,19 +133,30 @@ NP_HD uint32_t grid_nbr(uint32_t v, int rows, int cols, int k) {
 // graph: [log2(1+frontier)/24, log2(1+edges pending)/32, six class-group fractions,
 // min(it,256)/64, log2(n)/32].  Edges pending are estimated from the classes
 // (a class-c vertex has on average 0.75 * 2^c out-edges).
+// The group sums are six scalars updated by selects (adding 0 leaves a sum unchanged), not an
+// array indexed by the group, which a GPU thread would keep in scratch memory.
 NP_HD void features(const int *count, int it, int n, float *x) {
     float tot = 0.f, edges = 0.f;
-    float grp[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float g0 = 0.f, g1 = 0.f, g2 = 0.f, g3 = 0.f, g4 = 0.f, g5 = 0.f;
     for (int c = 1; c < NCLS; ++c) {
         float k = (float)count[c];
         tot += k;
         edges += k * 0.75f * (float)(1u << (c < 31 ? c : 31));
-        int g = c <= 2 ? 0 : c <= 4 ? 1 : c <= 6 ? 2 : c <= 8 ? 3 : c <= 12 ? 4 : 5;
-        grp[g] += k;
+        g0 += c <= 2 ? k : 0.f;
+        g1 += c > 2 && c <= 4 ? k : 0.f;
+        g2 += c > 4 && c <= 6 ? k : 0.f;
+        g3 += c > 6 && c <= 8 ? k : 0.f;
+        g4 += c > 8 && c <= 12 ? k : 0.f;
+        g5 += c > 12 ? k : 0.f;
     }
     x[0] = log2f(1.f + tot) / 24.f;
     x[1] = log2f(1.f + edges) / 32.f;
-    for (int g = 0; g < 6; ++g) x[2 + g] = tot > 0.f ? grp[g] / tot : 0.f;
+    x[2] = tot > 0.f ? g0 / tot : 0.f;
+    x[3] = tot > 0.f ? g1 / tot : 0.f;
+    x[4] = tot > 0.f ? g2 / tot : 0.f;
+    x[5] = tot > 0.f ? g3 / tot : 0.f;
+    x[6] = tot > 0.f ? g4 / tot : 0.f;
+    x[7] = tot > 0.f ? g5 / tot : 0.f;
     x[8] = (float)(it < 256 ? it : 256) / 64.f;
     x[9] = log2f((float)(n > 1 ? n : 1)) / 32.f;
 }

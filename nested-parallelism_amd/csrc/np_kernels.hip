@@ -130,21 +130,26 @@ struct PushCtx {
     int *q_next;        // push: next frontier queue
 };
 
-// Tail of one relaxation after the cheap check passed: the atomicMin of the reference
-// and the update mark.  Returns true when the caller must append tslot (push only).
+// Tail of one relaxation after the cheap check (alt < dist[tslot]) passed: the atomicMin of the
+// reference, issued without a return value, and the update mark.  Marking on the check alone
+// marks exactly the slots the atomic's old value would: when a concurrent atomicMin has already
+// stored less than alt, that writer passed its own check in this same launch and marks the slot
+// itself (every earlier store is visible to the check).  With a scanned frontier the mark is one
+// more atomic without a return, so no lane waits on either; the compaction drops vertices without
+// out-edges.  Pushing relaxers need the class of the slot and the bit's old value to append it
+// once.  Returns true when the caller must append tslot (push only).
 template <bool PUSH>
 __device__ __forceinline__ bool relax_tail(const DevState &st, const PushCtx &pc, int tslot,
                                            int alt, int &c_out) {
-    int old = atomicMin(&st.dist[tslot], alt);
-    if (alt >= old) return false;
-    int c = st.cls[tslot];
-    if (c == 0) return false;  // no out-edges: nothing to propagate
-    unsigned bit = 1u << (tslot & 31);
+    atomicMin(&st.dist[tslot], alt);
+    const unsigned bit = 1u << (tslot & 31);
     if (!PUSH) {
         atomicOr(&pc.bm_next[tslot >> 5], bit);
         return false;
     }
-    unsigned prev = atomicOr(&pc.bm_next[tslot >> 5], bit);
+    const int c = st.cls[tslot];
+    if (c == 0) return false;  // no out-edges: nothing to propagate
+    const unsigned prev = atomicOr(&pc.bm_next[tslot >> 5], bit);
     c_out = c;
     return (prev & bit) == 0;
 }
@@ -421,13 +426,13 @@ __device__ __forceinline__ void compaction_impl(unsigned *bm, const unsigned cha
         __syncthreads();
         const long long wi = wb + threadIdx.x;
         const unsigned word0 = wi < nwords ? ld_coherent_u(bm + wi) : 0u;
-        unsigned word = word0, take = 0;
+        unsigned word = word0, take = 0, drop = 0;
         while (word) {
             int b = __ffs(word) - 1;
             word &= word - 1;
             const long long slot = wi * 32 + b;
             int c = cls[slot];
-            if (!c) continue;
+            if (!c) { drop |= 1u << b; continue; }  // no out-edges: nothing to propagate
             if (T != NO_WINDOW) {
                 int d = ld_coherent(dist + slot);
                 if (d >= T) { my_dc++; my_dm = d < my_dm ? d : my_dm; continue; }
@@ -451,7 +456,7 @@ __device__ __forceinline__ void compaction_impl(unsigned *bm, const unsigned cha
             int p = atomicAdd(&s_lc[c], 1);
             st_coherent(q_out + seg_off[c] + s_lb[c] + p, (int)slot);
         }
-        if (take) st_coherent_u(bm + wi, word0 & ~take);
+        if (take | drop) st_coherent_u(bm + wi, word0 & ~(take | drop));
         __syncthreads();
     }
     if (T != NO_WINDOW) flush_deferred(my_dc, my_dm, &s_dc, &s_dm, defer_cnt, defer_min);
@@ -461,35 +466,45 @@ __device__ __forceinline__ void compaction_impl(unsigned *bm, const unsigned cha
 // schedule decision (all threads of a workgroup; the MLP is evaluated redundantly per
 // workgroup so every workgroup reaches the same decision without another barrier)
 // ------------------------------------------------------------------------------------
-__device__ __noinline__ Cfg decide(const SchedDev &sp, const int *cnt, int it) {
+// The MLP's decision, or the fixed schedule (level, klo, khi) when its output is unusable.
+// Called with scalars, not the SchedDev: a struct passed by reference to a function that is not
+// inlined is first stored to scratch memory by every caller.
+__device__ __noinline__ Cfg decide_mlp(const float *mlp, int n_feat, int flevel, int fklo, int fkhi,
+                                       const int *cnt, int it) {
+    static_assert(WG == 256 && NP_MLP_NIN <= 16, "decide() maps 32 outputs x 8 parts onto the workgroup");
     __shared__ float s_x[NP_MLP_NIN], s_h1[32], s_h2[32], s_o[5];
-    Cfg fixed = make_cfg(sp.level, sp.klo, sp.khi);
-    if (!sp.learned || !sp.mlp) return fixed;
-    if (threadIdx.x == 0) features(cnt, it, sp.n_feat, s_x);
-    __syncthreads();
-    const float *W1 = sp.mlp, *b1 = W1 + 32 * NP_MLP_NIN, *W2 = b1 + 32, *b2 = W2 + 32 * 32;
+    const Cfg fixed = make_cfg(flevel, fklo, fkhi);
+    const float *W1 = mlp, *b1 = W1 + 32 * NP_MLP_NIN, *W2 = b1 + 32, *b2 = W2 + 32 * 32;
     const float *W3 = b2 + 32, *b3 = W3 + 5 * 32;
-    const int j = threadIdx.x;
-    if (j < 32) {
-        float a = b1[j];
-#pragma unroll 2
-        for (int i = 0; i < NP_MLP_NIN; ++i) a += W1[j * NP_MLP_NIN + i] * s_x[i];
-        s_h1[j] = a > 0.f ? a : 0.f;
-    }
+    // Thread t computes part p = t % 8 of output j = t / 8 of each layer; the 8 parts of an output
+    // are adjacent lanes, summed with three shuffles.  Every weight is loaded before the first
+    // layer (W2 and W3 rows as float4, thread t at float 4t: one coalesced 4 KB read), so the
+    // forward pass waits for global memory once, not once per weight.
+    const int t = threadIdx.x, j = t >> 3, p = t & 7;
+    const float w1a = W1[j * NP_MLP_NIN + p];
+    const float w1b = p + 8 < NP_MLP_NIN ? W1[j * NP_MLP_NIN + p + 8] : 0.f;
+    const float4 w2 = *(const float4 *)(W2 + 4 * t);
+    const float4 w3 = t < 40 ? *(const float4 *)(W3 + 4 * t) : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float b1j = b1[j], b2j = b2[j], b3j = t < 40 ? b3[j] : 0.f;
+    if (t == 0) features(cnt, it, n_feat, s_x);
     __syncthreads();
-    if (j < 32) {
-        float a = b2[j];
-#pragma unroll 2
-        for (int i = 0; i < 32; ++i) a += W2[j * 32 + i] * s_h1[i];
-        s_h2[j] = a > 0.f ? a : 0.f;
-    }
+    float a = w1a * s_x[p] + (p + 8 < NP_MLP_NIN ? w1b * s_x[p + 8] : 0.f);
+    a += __shfl_xor(a, 1);
+    a += __shfl_xor(a, 2);
+    a += __shfl_xor(a, 4);
+    if (p == 0) s_h1[j] = a + b1j > 0.f ? a + b1j : 0.f;
     __syncthreads();
-    if (j < 5) {
-        float a = b3[j];
-#pragma unroll 2
-        for (int i = 0; i < 32; ++i) a += W3[j * 32 + i] * s_h2[i];
-        s_o[j] = a;
-    }
+    a = w2.x * s_h1[4 * p] + w2.y * s_h1[4 * p + 1] + w2.z * s_h1[4 * p + 2] + w2.w * s_h1[4 * p + 3];
+    a += __shfl_xor(a, 1);
+    a += __shfl_xor(a, 2);
+    a += __shfl_xor(a, 4);
+    if (p == 0) s_h2[j] = a + b2j > 0.f ? a + b2j : 0.f;
+    __syncthreads();
+    a = w3.x * s_h2[4 * p] + w3.y * s_h2[4 * p + 1] + w3.z * s_h2[4 * p + 2] + w3.w * s_h2[4 * p + 3];
+    a += __shfl_xor(a, 1);
+    a += __shfl_xor(a, 2);
+    a += __shfl_xor(a, 4);
+    if (t < 40 && p == 0) s_o[j] = a + b3j;
     __syncthreads();
     float o0 = s_o[0], o1 = s_o[1], o2 = s_o[2], o3 = s_o[3], o4 = s_o[4];
     __syncthreads();
@@ -500,19 +515,28 @@ __device__ __noinline__ Cfg decide(const SchedDev &sp, const int *cnt, int it) {
     return make_cfg(level, (int)rintf(o3), (int)rintf(o4));
 }
 
-__device__ __noinline__ void record_trace(const SchedDev &sp, int it, const Cfg &cfg, const int *cnt,
-                             long long t0, Ctrl *ctrl) {
-    if (!sp.trace || it >= sp.trace_cap) return;
-    np_trace_rec *r = sp.trace + it;
+__device__ __forceinline__ Cfg decide(const SchedDev &sp, const int *cnt, int it) {
+    if (!sp.learned || !sp.mlp) return make_cfg(sp.level, sp.klo, sp.khi);
+    return decide_mlp(sp.mlp, sp.n_feat, sp.level, sp.klo, sp.khi, cnt, it);
+}
+
+__device__ __noinline__ void record_trace_rec(np_trace_rec *r, int it, int level, int klo, int khi, const int *cnt,
+                                              long long t0, Ctrl *ctrl) {
     r->pairs = __hip_atomic_load(&ctrl->pairs_total, __ATOMIC_RELAXED, NP_AGENT);
     r->edges = __hip_atomic_load(&ctrl->edges_total, __ATOMIC_RELAXED, NP_AGENT);
     r->iteration = it;
-    r->level = cfg.level;
-    r->log2_t_lo = cfg.klo;
-    r->log2_t_hi = cfg.khi;
+    r->level = level;
+    r->log2_t_lo = klo;
+    r->log2_t_hi = khi;
     for (int c = 0; c < NCLS; ++c) r->count[c] = cnt[c];
     r->t_begin = t0;
     r->t_end = t0;
+}
+
+__device__ __forceinline__ void record_trace(const SchedDev &sp, int it, const Cfg &cfg, const int *cnt,
+                                             long long t0, Ctrl *ctrl) {
+    if (!sp.trace || it >= sp.trace_cap) return;
+    record_trace_rec(sp.trace + it, it, cfg.level, cfg.klo, cfg.khi, cnt, t0, ctrl);
 }
 
 // ------------------------------------------------------------------------------------
@@ -989,24 +1013,26 @@ constexpr int PACK_LANE_CLS = 10;
 // item -- target tv[k], pending source s, candidate alt[k] = dist[u][s] + w, loaded dist[v][s]
 // cur[k] -- or none (ok[k] false).  The lanes of one edge form a run of consecutive lanes (rs =
 // the run's first lane inside this wavefront, last = this lane ends its run), so they share v.
-// The reference's relaxation per item (sssp_kernel.cu:46-52): if (dist[v][s] > alt) atomicMin
-// (the UNR atomics are issued before any result is used); each improved source's bit then goes
-// to mflag[v], OR-ed across the run first so that one lane per run updates it, and the first bit
-// of mflag[v] sets v's frontier bit.  Vertices without out-edges are flagged too and dropped by
-// the compaction.  Every lane of the wavefront calls it together.
+// The reference's relaxation per item (sssp_kernel.cu:46-52): if (dist[v][s] > alt) atomicMin,
+// issued without a return value; each source that passed the check sets its bit in mflag[v],
+// OR-ed across the run first so that one lane per run updates it, and the first bit of mflag[v]
+// sets v's frontier bit.  Flagging on the check alone flags exactly the (v, s) the atomic's old
+// value would: when alt < cur but a concurrent atomicMin has already stored less, that writer
+// passed its own check in this same launch and flags (v, s) itself (every earlier store is
+// visible to the check).  So no lane waits for an atomic's return before the next step.
+// Vertices without out-edges are flagged too and dropped by the compaction.  Every lane of the
+// wavefront calls it together.
 __device__ __forceinline__ void pk_relax_steps(const DevState &st, const bool *ok, const int *tv, int s,
                                                const int *alt, const int *cur, int rs, bool last) {
-    int old[UNR];
 #pragma unroll
     for (int k = 0; k < UNR; ++k) {
-        old[k] = 0x7FFFFFFF;
         if (ok[k] && alt[k] < cur[k])
-            old[k] = atomicMin(st.dist + (((unsigned)tv[k] << st.kshift) + (unsigned)s), alt[k]);
+            atomicMin(st.dist + (((unsigned)tv[k] << st.kshift) + (unsigned)s), alt[k]);
     }
     const int lane = lane_id();
 #pragma unroll
     for (int k = 0; k < UNR; ++k) {
-        const unsigned bit = (ok[k] && alt[k] < cur[k] && alt[k] < old[k]) ? 1u << s : 0u;
+        const unsigned bit = (ok[k] && alt[k] < cur[k]) ? 1u << s : 0u;
         if (__ballot(bit != 0) == 0) continue;
         unsigned acc = bit;
 #pragma unroll
