@@ -1029,9 +1029,12 @@ __device__ __forceinline__ void pk_relax_steps(const DevState &st, const bool *o
         if (ok[k] && alt[k] < cur[k])
             atomicMin(st.dist + (((unsigned)tv[k] << st.kshift) + (unsigned)s), alt[k]);
     }
+    // every step's mflag OR is issued before any old value is tested: one wait for all UNR
     const int lane = lane_id();
+    unsigned prev[UNR];
 #pragma unroll
     for (int k = 0; k < UNR; ++k) {
+        prev[k] = 1u;
         const unsigned bit = (ok[k] && alt[k] < cur[k]) ? 1u << s : 0u;
         if (__ballot(bit != 0) == 0) continue;
         unsigned acc = bit;
@@ -1040,12 +1043,11 @@ __device__ __forceinline__ void pk_relax_steps(const DevState &st, const bool *o
             const unsigned y = __shfl_up(acc, d);
             if (lane - d >= rs) acc |= y;
         }
-        if (last && ok[k] && acc) {
-            const int v = tv[k];
-            const unsigned prev = atomicOr(&st.mflag[v], acc);
-            if (prev == 0) atomicOr(&st.bm[0][v >> 5], 1u << (v & 31));
-        }
+        if (last && ok[k] && acc) prev[k] = atomicOr(&st.mflag[tv[k]], acc);
     }
+#pragma unroll
+    for (int k = 0; k < UNR; ++k)
+        if (prev[k] == 0) atomicOr(&st.bm[0][tv[k] >> 5], 1u << (tv[k] & 31));
 }
 
 // Items of one queued vertex u -- edges [e0, e1) x the P pending sources of M -- on `stride`
