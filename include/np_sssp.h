@@ -51,7 +51,9 @@ enum {
     NP_ERR_NOT_CONVERGED = -5, /* iteration cap reached before the frontier emptied */
     NP_ERR_TIMEOUT = -6,       /* a grid barrier's bounded spin expired (kernel exited) */
     NP_ERR_UNSUPPORTED = -7,   /* no usable gfx950 device / feature */
-    NP_ERR_IO = -8             /* file could not be read or is malformed */
+    NP_ERR_IO = -8,            /* file could not be read or is malformed */
+    NP_ERR_BOUNDS = -9         /* bounds-checked build (libnpsssp_bounds.so) only: a frontier-queue or
+                                  delayed-buffer write fell outside its segment and was skipped */
 };
 
 /* Consolidation level (CONSOLIDATE_LEVEL: 0 warp, 1 block, 2 grid). */

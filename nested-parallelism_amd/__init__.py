@@ -21,7 +21,9 @@ import weakref
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libnpsssp.so")
+# NP_SSSP_LIB=bounds selects the bounds-checked build (libnpsssp_bounds.so: every frontier-queue
+# and delayed-buffer write checked against its segment, NP_ERR_BOUNDS on a violation)
+LIB_PATH = os.path.join(HERE, "libnpsssp_bounds.so" if os.environ.get("NP_SSSP_LIB") == "bounds" else "libnpsssp.so")
 
 INF = 1073741824          # common/global.h:19
 UNDEFINED = 0x7FFFFFFF    # apps/bfs-rec/src/gpu/bfs_rec.h:13
@@ -41,7 +43,8 @@ FRONTIER_SCAN, FRONTIER_PUSH = 0, 1
  SCHEME_WAVE_NESTED, SCHEME_WG_NESTED, SCHEME_BFS_FLAT, SCHEME_BFS_HIER) = range(9)
 
 ERRORS = {0: "NP_OK", -1: "NP_ERR_ARG", -2: "NP_ERR_HIP", -3: "NP_ERR_NOMEM", -4: "NP_ERR_CAPACITY",
-          -5: "NP_ERR_NOT_CONVERGED", -6: "NP_ERR_TIMEOUT", -7: "NP_ERR_UNSUPPORTED", -8: "NP_ERR_IO"}
+          -5: "NP_ERR_NOT_CONVERGED", -6: "NP_ERR_TIMEOUT", -7: "NP_ERR_UNSUPPORTED", -8: "NP_ERR_IO",
+          -9: "NP_ERR_BOUNDS"}
 
 
 class NpError(RuntimeError):

@@ -4,6 +4,8 @@
 
 Outputs next to this file:
     libnpsssp.so     HIP kernels + C ABI (include/np_sssp.h)
+    libnpsssp_bounds.so   the same with every frontier-queue and delayed-buffer write bounds-checked
+                          (NP_BOUNDS_CHECK=1; a violation is skipped and fails the solve with NP_ERR_BOUNDS)
     libnpdropin.so   the reference's C++ entry points (include/np_dropin.h) over the C ABI
     gpu-sssp, gpu-bfs-rec, gpu-bc, gpu-graph-color, gpu-pagerank, gpu-spmv   command-line drivers with the
                             reference's flags
@@ -41,20 +43,28 @@ def _obj(src):
     return os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
 
 
-def _compile(src):
+BOUNDS_SRC = ["np_kernels.hip", "np_api.cpp"]  # compiled a second time with NP_BOUNDS_CHECK=1
+
+
+def _compile(src, bounds=False):
     path = os.path.join(CSRC, src)
     obj = _obj(src)
+    extra = []
+    if bounds:
+        obj = obj[:-2] + "_bounds.o"
+        extra = ["-DNP_BOUNDS_CHECK=1"]
     if src.endswith(".hip"):
-        cmd = [HIPCC] + HIP_FLAGS + ["-c", path, "-o", obj]
+        cmd = [HIPCC] + HIP_FLAGS + extra + ["-c", path, "-o", obj]
     else:
-        cmd = [HIPCC] + HOST_FLAGS + ["-c", path, "-o", obj]
+        cmd = [HIPCC] + HOST_FLAGS + extra + ["-c", path, "-o", obj]
     return _run(cmd)
 
 
 def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
+    jobs = [(x, False) for x in HIP_SRC + HOST_SRC + ["np_dropin.cpp"]] + [(x, True) for x in BOUNDS_SRC]
     with ThreadPoolExecutor(max_workers=4) as ex:
-        outs = list(ex.map(_compile, HIP_SRC + HOST_SRC + ["np_dropin.cpp"]))
+        outs = list(ex.map(lambda j: _compile(*j), jobs))
     if verbose:
         for o in outs:
             if o.strip():
@@ -63,6 +73,10 @@ def build(verbose=False):
     tmp = lib + ".tmp"
     _run([HIPCC] + ARCH + ["-shared", "-fPIC", "-o", tmp] + [_obj(s) for s in HIP_SRC + HOST_SRC])
     os.replace(tmp, lib)
+    blib = os.path.join(HERE, "libnpsssp_bounds.so")
+    bobjs = [_obj(s)[:-2] + "_bounds.o" if s in BOUNDS_SRC else _obj(s) for s in HIP_SRC + HOST_SRC]
+    _run([HIPCC] + ARCH + ["-shared", "-fPIC", "-o", blib + ".tmp"] + bobjs)
+    os.replace(blib + ".tmp", blib)
     dropin = os.path.join(HERE, "libnpdropin.so")
     _run([HIPCC, "-shared", "-fPIC", "-o", dropin + ".tmp", _obj("np_dropin.cpp"),
           "-L" + HERE, "-lnpsssp", "-Wl,-rpath,$ORIGIN"])

@@ -23,6 +23,9 @@ namespace np {
 
 static thread_local char g_err[512] = "";
 
+static const char *BOUNDS_MSG = "bounds-checked build: a frontier-queue or delayed-buffer write fell outside its "
+                                "segment (skipped; the distances are not valid)";
+
 void set_error(const char *fmt, ...) {
     va_list ap;
     va_start(ap, fmt);
@@ -458,6 +461,7 @@ static int run_drive(np_ctx *c, Space &s, const DevState &st, const np_sched &sc
     rc = h.error ? h.error : NP_OK;
     if (rc == NP_ERR_TIMEOUT) set_error("a grid barrier timed out; the kernel exited early");
     if (rc == NP_ERR_NOT_CONVERGED) set_error("iteration cap reached");
+    if (rc == NP_ERR_BOUNDS) set_error("%s", BOUNDS_MSG);
     if (rc == NP_OK && !unit) rc = distance_guard(c, s, st, batch);
     if (stats) {
         std::memset(stats, 0, sizeof(*stats));
@@ -656,7 +660,11 @@ static void batch_release(np_batch *b) {
 extern "C" {
 
 const char *np_last_error(void) { return g_err; }
-const char *np_version(void) { return "np-sssp gfx950 1.0"; }
+#ifndef NP_BOUNDS_CHECK
+#define NP_BOUNDS_CHECK 0
+#endif
+const char *np_version(void) { return NP_BOUNDS_CHECK ? "np-sssp gfx950 1.0 bounds-checked" : "np-sssp gfx950 1.0"; }
+
 
 // ------------------------------------------------------------------------------------
 // allocation-free single-source solves (np_sssp_device_noalloc)
@@ -1733,6 +1741,7 @@ static int run_parts_concurrent(np_ctx *c, std::vector<PartJob> &jobs, const np_
     }
     if (rc == NP_ERR_NOT_CONVERGED) set_error("iteration cap reached");
     if (rc == NP_ERR_TIMEOUT) set_error("a grid barrier timed out; the kernel exited early");
+    if (rc == NP_ERR_BOUNDS) set_error("%s", BOUNDS_MSG);
     for (PartJob &j : jobs)
         if (rc == NP_OK && !unit) rc = distance_guard(c, *j.s, j.st, j.batch);
     return rc;

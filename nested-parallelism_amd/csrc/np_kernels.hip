@@ -53,6 +53,24 @@ __device__ __forceinline__ void st_coherent_u(unsigned *p, unsigned v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, NP_AGENT);
 }
 
+#ifndef NP_BOUNDS_CHECK
+#define NP_BOUNDS_CHECK 0
+#endif
+// Bounds-checked build (NP_BOUNDS_CHECK=1, libnpsssp_bounds.so): every write into a frontier
+// queue, its source masks or a scheme's delayed buffer checks its index against the segment it
+// belongs to, [lo, hi); an index outside is not written and the solve fails with NP_ERR_BOUNDS.
+// The normal build compiles the check out.
+__device__ __forceinline__ bool seg_ok(Ctrl *ctrl, long long idx, long long lo, long long hi) {
+#if NP_BOUNDS_CHECK
+    if (idx >= lo && idx < hi) return true;
+    atomicCAS(&ctrl->error, 0, (int)NP_ERR_BOUNDS);
+    return false;
+#else
+    (void)ctrl; (void)idx; (void)lo; (void)hi;
+    return true;
+#endif
+}
+
 __device__ __forceinline__ int wave_incl_scan(int x) {
     const int lane = lane_id();
 #pragma unroll
@@ -168,7 +186,8 @@ __device__ __forceinline__ void push_wave(const DevState &st, const PushCtx &pc,
         base = __shfl(base, leader);
         if (want && c == cl) {
             int pos = base + __popcll(same & __lanemask_lt());
-            st_coherent(pc.q_next + st.seg_off[cl] + pos, slot);
+            if (seg_ok(st.ctrl, (long long)st.seg_off[cl] + pos, st.seg_off[cl], st.seg_off[cl + 1]))
+                st_coherent(pc.q_next + st.seg_off[cl] + pos, slot);
             want = false;
         }
         mask &= ~same;
@@ -414,7 +433,7 @@ __device__ __forceinline__ void flush_deferred(int my_dc, int my_dm, int *s_dc, 
     if (threadIdx.x == 0 && *s_dc) { atomicAdd(defer_cnt, *s_dc); atomicMin(defer_min, *s_dm); }
 }
 
-__device__ __forceinline__ void compaction_impl(unsigned *bm, const unsigned char *cls, const int *seg_off,
+__device__ __forceinline__ void compaction_impl(Ctrl *ctrl, unsigned *bm, const unsigned char *cls, const int *seg_off,
                                                 long long nwords, int *cnt_out, int *q_out, const int *dist,
                                                 int T, int *defer_cnt, int *defer_min) {
     __shared__ int s_lc[NCLS], s_lb[NCLS];
@@ -454,7 +473,8 @@ __device__ __forceinline__ void compaction_impl(unsigned *bm, const unsigned cha
             long long slot = wi * 32 + b;
             int c = cls[slot];
             int p = atomicAdd(&s_lc[c], 1);
-            st_coherent(q_out + seg_off[c] + s_lb[c] + p, (int)slot);
+            if (seg_ok(ctrl, (long long)seg_off[c] + s_lb[c] + p, seg_off[c], seg_off[c + 1]))
+                st_coherent(q_out + seg_off[c] + s_lb[c] + p, (int)slot);
         }
         if (take | drop) st_coherent_u(bm + wi, word0 & ~(take | drop));
         __syncthreads();
@@ -571,7 +591,7 @@ __global__ __launch_bounds__(WG) void k_persistent(DevState st, SchedDev sp) {
                     st_coherent(ctrl->defer_cnt + (pass + 1) % 3, 0);
                     st_coherent(ctrl->defer_min + (pass + 1) % 3, NO_WINDOW);
                 }
-                compaction_impl(st.bm[0], st.cls, st.seg_off, st.nwords, cnt_cur, q_cur, st.dist, T, dc, dm);
+                compaction_impl(st.ctrl, st.bm[0], st.cls, st.seg_off, st.nwords, cnt_cur, q_cur, st.dist, T, dc, dm);
                 if (!grid_sync(ctrl, nb, gen)) return;
                 pass++;
             }
@@ -632,7 +652,7 @@ __global__ __launch_bounds__(WG) void k_compact(DevState st, int it_off) {
         ctrl->defer_cnt[(pass + 1) % 3] = 0;
         ctrl->defer_min[(pass + 1) % 3] = NO_WINDOW;
     }
-    compaction_impl(st.bm[0], st.cls, st.seg_off, st.nwords, st.cnt + (it % 3) * NCLS, st.queue[it & 1],
+    compaction_impl(st.ctrl, st.bm[0], st.cls, st.seg_off, st.nwords, st.cnt + (it % 3) * NCLS, st.queue[it & 1],
                     st.dist, T, ctrl->defer_cnt + pass % 3, ctrl->defer_min + pass % 3);
 }
 
@@ -853,7 +873,8 @@ __global__ __launch_bounds__(WG) void k_sol_parent(DevState st, SolBufs sb, int 
                 int at = 0;
                 if (lane == 0) at = atomicAdd(sb.bcnt + it % 3, __popcll(m));
                 at = __shfl(at, 0);
-                if (heavy) st_coherent(sb.buf + at + __popcll(m & __lanemask_lt()), slot);
+                const long long bi = (long long)at + __popcll(m & __lanemask_lt());
+                if (heavy && seg_ok(ctrl, bi, 0, st.seg_off[NCLS])) st_coherent(sb.buf + bi, slot);
             }
         }
     }
@@ -1336,8 +1357,10 @@ __device__ __forceinline__ void pk_compaction(const DevState &st, int *cnt_out, 
             const long long qi = st.seg_off[c] + s_lb[c] + atomicAdd(&s_lc[c], 1);
             const unsigned t = ld_coherent_u(st.mflag + v);
             st_coherent_u(st.mflag + v, 0u);
-            st_coherent(q_out + qi, (int)v);
-            st_coherent_u(qm_out + qi, t);
+            if (seg_ok(st.ctrl, qi, st.seg_off[c], st.seg_off[c + 1])) {
+                st_coherent(q_out + qi, (int)v);
+                st_coherent_u(qm_out + qi, t);
+            }
             pairs += __popc(t);
         }
         if (word) st_coherent_u(bm + wi, 0u);
@@ -1384,8 +1407,10 @@ __device__ __forceinline__ void pk_compaction(const DevState &st, int *cnt_out, 
             if (!qc[k]) continue;
             const long long qi = st.seg_off[qc[k]] + s_lb[qc[k]] + qp[k];
             const long long v = vb + k * WG + threadIdx.x;
-            st_coherent(q_out + qi, (int)v);
-            st_coherent_u(qm_out + qi, qt[k]);
+            if (seg_ok(st.ctrl, qi, st.seg_off[qc[k]], st.seg_off[qc[k] + 1])) {
+                st_coherent(q_out + qi, (int)v);
+                st_coherent_u(qm_out + qi, qt[k]);
+            }
             pairs += __popc(qt[k]);
         }
     }

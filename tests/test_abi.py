@@ -32,6 +32,19 @@ def test_c_abi_exports_every_declared_symbol(npa):
     assert sorted(npa.EXPORTS) == declared
 
 
+def test_bounds_checked_build_exports_the_same_abi():
+    """libnpsssp_bounds.so (every frontier-queue and delayed-buffer write checked against its
+    segment) is a drop-in for libnpsssp.so: the same exports, and it says which build it is."""
+    blib = os.path.join(PKG_DIR, "libnpsssp_bounds.so")
+    syms = _exported(blib)
+    missing = [d for d in _declared("np_sssp.h") if d not in syms]
+    assert not missing, missing
+    for path, tag in ((blib, True), (os.path.join(PKG_DIR, "libnpsssp.so"), False)):
+        L = ctypes.CDLL(path)
+        L.np_version.restype = ctypes.c_char_p
+        assert (b"bounds-checked" in L.np_version()) == tag, path
+
+
 def test_dropin_exports_reference_names():
     syms = _exported(os.path.join(PKG_DIR, "libnpdropin.so"))
     # C++ names of common/global.h and apps/sssp/src/gpu/sssp.h / bfs_rec.h
