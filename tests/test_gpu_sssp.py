@@ -373,6 +373,43 @@ def test_packed_batch(npa, ctx, orc):
         g.free()
 
 
+def _hub_graph(rng, n, hubs, deg, extra_m):
+    """Random graph plus `hubs` vertices of `deg` out-edges each (edge lists longer than one
+    2048-edge grid chunk and than several 256-edge neighbour tiles)."""
+    src = np.concatenate([rng.integers(0, n, extra_m), np.repeat(np.arange(hubs), deg)])
+    dst = np.concatenate([rng.integers(0, n, extra_m), rng.integers(0, n, hubs * deg)])
+    w = rng.integers(1, 200, len(src))
+    order = np.argsort(src, kind="stable")
+    src, dst, w = src[order], dst[order], w[order]
+    row = np.zeros(n + 1, np.int64)
+    np.add.at(row, src + 1, 1)
+    return np.cumsum(row).astype(np.int32), dst.astype(np.int32), w.astype(np.int32)
+
+
+def test_packed_hub_tiles(npa, ctx, orc):
+    """Source packs of 2, 3, 17 and 32 on a graph with hubs of 2,049 and 5,003 out-edges: the
+    wavefront and grid bins stage each vertex's edges in LDS tiles rounded to whole groups of
+    four steps (4 * floor(64 / P) edges: 252 for P = 3 and 17, 256 for 2 and 32), so tile, chunk
+    and group ends fall everywhere in the hub rows; every level and drive, exact."""
+    rng = np.random.default_rng(71)
+    row, col, w = _hub_graph(rng, 6000, 3, 2049, 30000)
+    row2, col2, w2 = _hub_graph(rng, 5000, 2, 5003, 20000)
+    for r, c, ww in ((row, col, w), (row2, col2, w2)):
+        G = npa.Graph.from_csr(ctx, r, c, ww)
+        srcs = [int(s) for s in npa.pick_sources(r, 32, 71)]
+        refs = [orc.dijkstra(r, c, ww, s)[0] for s in srcs]
+        for k in (2, 3, 17, 32):
+            B = npa.Batch(ctx, [G] * k, srcs[:k])
+            assert B.layout() == (1, 0)
+            for drive in (0, 2):
+                for lv, lo, hi in ((2, 4, 8), (0, 4, 4), (1, 4, 12), (2, 0, 11)):
+                    B.solve(_sched(npa, lv, lo, hi, drive, 0))
+                    for i in range(k):
+                        assert np.array_equal(B.get(i), refs[i]), (k, drive, lv, lo, hi, i)
+            B.free()
+        G.free()
+
+
 def test_pack_max_option(npa, ctx, orc):
     """NP_CTX_PACK_MAX caps the sources per pack: 20 sources on one graph go to ceil(20 / max)
     packs of near-equal size, each solved exactly; values outside 2..32 are refused."""
