@@ -22,7 +22,8 @@ same path (ER 64K, R-MAT s22 per level, many sources + many uneven graphs, grid 
 graph); they print the same JSON schema.  --config bc measures betweenness centrality
 (BC_GPU) on the same frontier machinery (R-MAT s16, 1024 sources); --config color the graph
 coloring of GRAPH_COLOR_GPU (R-MAT s20); --config pagerank PAGERANK_GPU (R-MAT s20); --config
-spmv SpMV_GPU (R-MAT s22).
+spmv SpMV_GPU (R-MAT s22); --config bfs the BFS levels of BFS_REC_GPU on the R-MAT s24 graph and
+16 sources of c3 (the reference's bfs-rec solutions, the consolidation levels and a packed batch).
 
 The CPU leg (skipped with --no-cpu) is the checker and the CPU baseline: every distance of
 every instance and mode is checked with the shortest-path certificate, a bounded sample is
@@ -392,6 +393,35 @@ def pr_bench(npa, ctx, args):
 SPMV_METRIC = "SpMV GFLOP/s + %HBM peak @R-MAT s22 (float64), exact"
 
 
+def copy_calibration(npa, ctx, nbytes):
+    """Three device-to-device copies of nbytes (a copy kernel of known size for scaling the
+    FETCH_SIZE / WRITE_SIZE counters of a profiled run); nothing when nbytes is 0."""
+    if nbytes <= 0:
+        return
+    n4 = nbytes // 4
+    a, b = npa.DeviceBuffer(ctx, n4), npa.DeviceBuffer(ctx, n4)
+    for _ in range(3):
+        npa._check(npa.load().np_dev_copy(ctx.h, npa.ctypes.c_void_p(b.ptr), npa.ctypes.c_void_p(a.ptr), n4 * 4, 2))
+    log("copy calibration: 3 device-to-device copies of %d bytes" % (n4 * 4))
+    a.free()
+    b.free()
+
+
+def stored_traffic(config, head):
+    """(bytes, note) of the mode `head` from profiles/traffic_<config>.json, or (None, None)."""
+    tpath = os.path.join(ROOT, "profiles", "traffic_%s.json" % config)
+    try:
+        tj = json.load(open(tpath))
+    except (ValueError, OSError):
+        return None, None
+    rec = tj.get(head) or (tj.get("batched") if head == "learned_batched" else None)
+    if not isinstance(rec, dict):
+        return None, None
+    return rec.get("bytes"), {"source": "stored, not measured in this run: profiles/traffic_%s.json" % config,
+                              "unscaled_bytes": rec.get("unscaled_bytes"), "fetch_scale": rec.get("fetch_scale"),
+                              "write_scale": rec.get("write_scale"), "profile": rec.get("profile")}
+
+
 def measured_traffic(config):
     """HBM bytes per launch measured with rocprofv3 counters (profiles/traffic_<config>.json), or None."""
     tpath = os.path.join(ROOT, "profiles", "traffic_%s.json" % config)
@@ -447,6 +477,112 @@ def spmv_bench(npa, ctx, args):
     G.free()
 
 
+BFS_METRIC = "BFS_REC_GPU GTEPS + %HBM peak @R-MAT s24, 16 sources, exact"
+
+
+def bfs_bench(npa, ctx, args):
+    """--config bfs: BFS_REC_GPU levels (the same frontier machinery with unit lengths) on the
+    R-MAT scale-24 graph of the SSSP configuration (16,777,216 vertices, 268,435,456 edges, ids
+    permuted), from its 16 sources (seed 3, out-degree >= 1); a step solves every source, levels
+    left on the device (np_bfs_device).  Modes: the reference's bfs-rec solutions 0 (flat: every
+    vertex at every level), 1 (naive recursion: a workgroup per frontier vertex, children pushed
+    by the relaxers) and 2 (hierarchical: two levels per expansion) on the host loop as the
+    reference runs them; solution 3 (consolidation) at levels 0/1/2 and level 2 on the graph
+    drive; and the 16 sources source-packed in one batch.  GTEPS = sum over sources of the
+    out-edges of reached vertices / step time.  Algorithmic bytes per source = 8 per reached
+    edge (col 4 + the level gather 4; no weights) + 16 per reached vertex (row bounds 8, own
+    level 4, final level write 4).  Every level array of every mode is compared with the
+    oracle's BFS (restated bfs_cpu), which is also the CPU baseline."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    t0 = time.time()
+    W = build_workload(npa, ctx, "c3", args)
+    G = W.graphs[0]
+    srcs = [s for _, s in W.inst]
+    row, col, _ = G.download()
+    deg = np.diff(row).astype(np.int64)
+    log("bfs: %s, built in %.1fs" % (W.desc, time.time() - t0))
+    levels = [ctx.bfs(G, s)[0] for s in srcs]
+    reached = [lv != npa.UNDEFINED for lv in levels]
+    n_r = np.array([int(r.sum()) for r in reached], np.int64)
+    m_r = np.array([int(deg[r].sum()) for r in reached], np.int64)
+    total_m = int(m_r.sum())
+    b_alg = (8 * m_r + 16 * n_r).astype(np.float64)
+    scheds = {
+        "sol0_flat": npa.Sched(level=1, scheme=npa.SCHEME_BFS_FLAT, drive=npa.DRIVE_HOSTLOOP),
+        "sol1_push": npa.Sched(level=1, log2_t_lo=0, log2_t_hi=0, frontier=npa.FRONTIER_PUSH,
+                               drive=npa.DRIVE_HOSTLOOP),
+        "sol2_hier": npa.Sched(level=1, scheme=npa.SCHEME_BFS_HIER, drive=npa.DRIVE_HOSTLOOP),
+        "level0": npa.Sched(level=0), "level1": npa.Sched(level=1), "level2": npa.Sched(level=2),
+        "level2_graph": npa.Sched(level=2, log2_t_lo=4, log2_t_hi=10, drive=npa.DRIVE_GRAPH),
+        "batched": npa.Sched(level=2, log2_t_lo=4, log2_t_hi=10, drive=npa.DRIVE_GRAPH),
+    }
+    modes = [m for m in (args.modes.split(",") if args.modes else scheds) if m in scheds]
+    d_lev = npa.DeviceBuffer(ctx, G.n)
+    B = npa.Batch(ctx, [G] * len(srcs), srcs, unit=True) if "batched" in modes else None
+    results, finals = {}, {}
+    for mode in modes:
+        sch = scheds[mode]
+        if mode == "batched":
+            def step(sch=sch):
+                return [B.solve(sch).kernel_ms]
+        else:
+            def step(sch=sch):
+                return [ctx.bfs_device(G, s, d_lev.ptr, sch).kernel_ms for s in srcs]
+        ms, launch = time_mode(ctx, step, args.steps, args.warmup)
+        results[mode] = {"gteps": total_m / (ms * 1e-3) / 1e9, "ms_per_step": ms,
+                         "launch_ms": float(np.mean(launch)), "launches_per_step": len(launch) // args.steps}
+        finals[mode] = [B.get(i) for i in range(len(srcs))] if mode == "batched" else \
+            [ctx.bfs(G, s, sch)[0] for s in srcs]
+        log("%-12s %9.2f ms/step %8.3f GTEPS  (%d launches/step, %.3f ms each)  %r" % (
+            mode, ms, results[mode]["gteps"], results[mode]["launches_per_step"], results[mode]["launch_ms"], sch))
+    exact = cpu = None
+    if not args.no_cpu:
+        t = time.perf_counter()
+        ref = [oracle.bfs_levels(row, col, s) for s in srcs]
+        tc = time.perf_counter() - t
+        exact = all(np.array_equal(f[i], ref[i]) for f in finals.values() for i in range(len(srcs))) and \
+            all(np.array_equal(levels[i], ref[i]) for i in range(len(srcs)))
+        model, nproc, _ = host_cpu()
+        cpu = {"value": round(total_m / tc / 1e9, 5), "unit": "GTEPS", "cores": 1, "kind": "port", "nproc": nproc,
+               "cpu_model": model, "aggregate_s": round(tc, 3), "per_source_s": round(tc / len(srcs), 3),
+               "sample": "oracle queue BFS (restated bfs_cpu, bfs_rec.cpp:100-119) over all %d sources in sequence "
+                         "on one thread, %.1f s; every level of every GPU mode equal: %s" % (len(srcs), tc, exact)}
+        log("cpu baseline: %.5f GTEPS on one thread (%.1f s for %d sources); exact %s" % (cpu["value"], tc, len(srcs), exact))
+    head = max(results, key=lambda m: results[m]["gteps"])
+    hr = results[head]
+    per_launch_bytes = float(b_alg.sum()) if head == "batched" else float(b_alg.mean())
+    achieved = per_launch_bytes / (hr["launch_ms"] * 1e-3) / 1e9
+    traffic, traffic_note = stored_traffic("bfs", head)
+    drive = scheds[head].drive
+    kernel = ("k_pk_compact+k_decide+k_pk_expand" if head == "batched" else
+              {"sol0_flat": "k_bfs_flat", "sol2_hier": "k_bfs_hier"}.get(head, "k_persistent" if drive == 0 else
+                                                                          "k_compact+k_decide+k_expand"))
+    kernel += " (%s drive)" % {0: "persistent", 1: "host-loop", 2: "graph"}[drive]
+    line = {
+        "metric": BFS_METRIC, "value": round(hr["gteps"], 3), "unit": "GTEPS", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(hr["ms_per_step"], 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic",
+        "config": {"workload": "BFS levels (BFS_REC_GPU), " + W.desc.replace("SSSP, ", "", 1) +
+                               " (weights unused); one step solves every source",
+                   "path": head, "schedule": repr(scheds[head]), "n": int(G.n), "m": int(G.m),
+                   "sources": len(srcs), "reached_edges_per_step": total_m},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_note": traffic_note,
+                     "kernel": kernel, "bytes_per_launch": per_launch_bytes},
+        "cpu_baseline": cpu, "exact": exact,
+        "modes": {k: {"gteps": round(v["gteps"], 3), "ms_per_step": round(v["ms_per_step"], 3),
+                      "launch_ms": round(v["launch_ms"], 4), "schedule": repr(scheds[k])} for k, v in results.items()},
+    }
+    print(json.dumps(line), flush=True)
+    copy_calibration(npa, ctx, args.copy_calibration)
+    if B is not None:
+        B.free()
+    d_lev.free()
+    G.free()
+    return 1 if exact is False else 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -495,6 +631,8 @@ def run_config(npa, ctx, args):
     if args.config == "spmv":
         spmv_bench(npa, ctx, args)
         return 0
+    if args.config == "bfs":
+        return bfs_bench(npa, ctx, args)
     t0 = time.time()
     W = build_workload(npa, ctx, args.config, args)
     log("%s: %d graphs, %d instances, built in %.1fs" % (W.desc, len(W.graphs), len(W.inst), time.time() - t0))
@@ -620,19 +758,7 @@ def run_config(npa, ctx, args):
         per_launch_bytes = float(b_alg.mean())
     kernel += " (%s drive)" % {0: "persistent", 1: "host-loop", 2: "graph"}[scheds[head].drive]
     achieved = per_launch_bytes / (hr["launch_ms"] * 1e-3) / 1e9
-    traffic, traffic_note = None, None
-    tpath = os.path.join(ROOT, "profiles", "traffic_%s.json" % args.config)
-    if os.path.exists(tpath):
-        try:
-            tj = json.load(open(tpath))
-            rec = tj.get(head) or (tj.get("batched") if head == "learned_batched" else None)
-            if isinstance(rec, dict):
-                traffic = rec.get("bytes")
-                traffic_note = {"source": "stored, not measured in this run: profiles/traffic_%s.json" % args.config,
-                                "unscaled_bytes": rec.get("unscaled_bytes"), "fetch_scale": rec.get("fetch_scale"),
-                                "write_scale": rec.get("write_scale"), "profile": rec.get("profile")}
-        except (ValueError, OSError):
-            traffic = None
+    traffic, traffic_note = stored_traffic(args.config, head)
     # work of one traced solve of the fastest mode (not timed): edge rows read (out-degrees of the queued
     # vertices), (edge, source) items and queued (vertex, source) pairs; for packs the mean number of
     # sources pending per edge row read and the re-expansion of pairs over the reached ones
@@ -682,15 +808,7 @@ def run_config(npa, ctx, args):
                                      if "learned" in results and "tuned" in results else None),
     }
     print(json.dumps(line), flush=True)
-    if args.copy_calibration > 0:
-        n4 = args.copy_calibration // 4
-        a, b = npa.DeviceBuffer(ctx, n4), npa.DeviceBuffer(ctx, n4)
-        for _ in range(3):
-            npa._check(npa.load().np_dev_copy(ctx.h, npa.ctypes.c_void_p(b.ptr), npa.ctypes.c_void_p(a.ptr),
-                                              n4 * 4, 2))
-        log("copy calibration: 3 device-to-device copies of %d bytes" % (n4 * 4))
-        a.free()
-        b.free()
+    copy_calibration(npa, ctx, args.copy_calibration)
     for B in batches.values():
         B.free()
     dbuf.free()

@@ -249,6 +249,11 @@ int np_sssp_device_noalloc(np_ctx *ctx, np_graph *g, int32_t src, const np_sched
  * unreached = NP_UNDEFINED.  level_out: host int32[n]. */
 int np_bfs(np_ctx *ctx, np_graph *g, int32_t src, const np_sched *sched, int32_t *level_out,
            np_stats *stats);
+/* np_bfs with the levels left in device memory (d_level: device int32[n]), as np_sssp_device
+ * is to np_sssp.  No separate reference counterpart: BFS_REC_GPU (bfs_rec_wrapper.cu:231-269)
+ * always copies levelArray back; this form lets callers keep levels resident. */
+int np_bfs_device(np_ctx *ctx, np_graph *g, int32_t src, const np_sched *sched, int32_t *d_level,
+                  np_stats *stats);
 
 /* Batched solves -- no counterpart in the reference.  `count` instances; instance i
  * solves graph graphs[i] from source srcs[i] (graphs may repeat: many sources on one

@@ -126,6 +126,7 @@ _SIGS = [
     ("np_sssp_device_noalloc", ctypes.c_int, [_vp, _vp, ctypes.c_int32, ctypes.POINTER(Sched), _vp,
                                               ctypes.POINTER(Stats)]),
     ("np_bfs", ctypes.c_int, [_vp, _vp, ctypes.c_int32, ctypes.POINTER(Sched), _vp, ctypes.POINTER(Stats)]),
+    ("np_bfs_device", ctypes.c_int, [_vp, _vp, ctypes.c_int32, ctypes.POINTER(Sched), _vp, ctypes.POINTER(Stats)]),
     ("np_batch_create", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.POINTER(_vp), _vp, ctypes.c_int32,
                                        ctypes.POINTER(_vp)]),
     ("np_batch_solve", ctypes.c_int, [_vp, _vp, ctypes.POINTER(Sched), ctypes.POINTER(Stats)]),
@@ -419,6 +420,13 @@ class Context:
         _check(load().np_bfs(self.h, g.h, int(src), ctypes.byref(sched or Sched(level=2)), _ptr(lev),
                              ctypes.byref(st)))
         return lev, st
+
+    def bfs_device(self, g, src, d_level_ptr, sched=None):
+        """BFS levels into device memory (np_bfs_device).  Returns Stats."""
+        st = Stats()
+        _check(load().np_bfs_device(self.h, g.h, int(src), ctypes.byref(sched or Sched(level=2)),
+                                    ctypes.c_void_p(d_level_ptr), ctypes.byref(st)))
+        return st
 
     def bc(self, g, srcs=None):
         """Betweenness centrality (BC_GPU, solution 7) summed over `srcs` (None: every vertex).

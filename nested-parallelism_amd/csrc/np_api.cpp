@@ -1030,6 +1030,11 @@ int np_bfs(np_ctx *c, np_graph *g, int32_t src, const np_sched *sched, int32_t *
     return solve_single(c, g, src, sched, nullptr, level_out, true, stats);
 }
 
+int np_bfs_device(np_ctx *c, np_graph *g, int32_t src, const np_sched *sched, int32_t *d_level, np_stats *stats) {
+    if (!d_level) { set_error("null d_level"); return NP_ERR_ARG; }
+    return solve_single(c, g, src, sched, d_level, nullptr, true, stats);
+}
+
 int np_reached_work(np_ctx *c, np_graph *g, const int32_t *d_dist, int32_t inf, int64_t *n_r, int64_t *m_r) {
     if (!c || !g || !d_dist) { set_error("null argument"); return NP_ERR_ARG; }
     if (g->ctx != c) { set_error("graph belongs to another context or its context was destroyed"); return NP_ERR_ARG; }

@@ -410,6 +410,32 @@ def test_packed_hub_tiles(npa, ctx, orc):
         G.free()
 
 
+def test_bfs_device_matches_bfs(npa, ctx, orc):
+    """np_bfs_device leaves the levels of np_bfs in device memory: same levels, on a renumbered
+    R-MAT graph and a plain random graph, for the reference's bfs-rec solutions and the levels."""
+    rng = np.random.default_rng(81)
+    graphs = [npa.Graph.rmat(ctx, 14, 16, wmin=1, wmax=255, seed=81),
+              npa.Graph.from_csr(ctx, *random_csr(rng, 7000, 50000, 1, 9)[:3])]
+    scheds = [npa.Sched(level=lv) for lv in (0, 1, 2)] + [
+        npa.Sched(level=1, scheme=npa.SCHEME_BFS_FLAT, drive=npa.DRIVE_HOSTLOOP),
+        npa.Sched(level=1, scheme=npa.SCHEME_BFS_HIER, drive=npa.DRIVE_HOSTLOOP),
+        npa.Sched(level=1, log2_t_lo=0, log2_t_hi=0, frontier=npa.FRONTIER_PUSH, drive=npa.DRIVE_HOSTLOOP),
+        npa.Sched(level=2, log2_t_lo=4, log2_t_hi=10, drive=npa.DRIVE_GRAPH)]
+    for G in graphs:
+        row, col, _ = G.download()
+        s = int(npa.pick_sources(row, 1, 81)[0])
+        ref = orc.bfs_levels(row, col, s)
+        d = npa.DeviceBuffer(ctx, G.n)
+        for sch in scheds:
+            lev, _ = ctx.bfs(G, s, sch)
+            st = ctx.bfs_device(G, s, d.ptr, sch)
+            assert st.iterations > 0
+            assert np.array_equal(lev, ref), sch
+            assert np.array_equal(d.to_host(), ref), sch
+        d.free()
+        G.free()
+
+
 def test_pack_max_option(npa, ctx, orc):
     """NP_CTX_PACK_MAX caps the sources per pack: 20 sources on one graph go to ceil(20 / max)
     packs of near-equal size, each solved exactly; values outside 2..32 are refused."""
