@@ -595,7 +595,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--modes", default=None)
     ap.add_argument("--sched", action="append", default=[],
-                    help="MODE=level,log2_t_lo,log2_t_hi,wgs_per_cu[,drive[,delta]]: use this schedule for MODE (tuned or "
+                    help="MODE=level,log2_t_lo,log2_t_hi,wgs_per_cu[,drive[,delta[,pack_hold]]]: use this schedule for MODE (tuned or "
                          "batched) instead of running the autotuner (profiling runs hold only timed launches)")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="after the run, copy this many bytes device-to-device 3 times (a copy kernel of known "
@@ -652,7 +652,8 @@ def run_config(npa, ctx, args):
         mode, vals = spec.split("=")
         v = [int(x) for x in vals.split(",")]
         fixed[mode] = npa.Sched(level=v[0], log2_t_lo=v[1], log2_t_hi=v[2], wgs_per_cu=v[3],
-                                drive=v[4] if len(v) > 4 else npa.DRIVE_PERSISTENT, delta=v[5] if len(v) > 5 else 0)
+                                drive=v[4] if len(v) > 4 else npa.DRIVE_PERSISTENT, delta=v[5] if len(v) > 5 else 0,
+                                pack_hold=v[6] if len(v) > 6 else 0)
     if "tuned" in fixed:
         scheds["tuned"] = fixed["tuned"]
     elif "tuned" in modes or "concat" in modes or "learned" in modes:

@@ -95,7 +95,12 @@ typedef struct np_sched {
                             vertices with dist < T are expanded, T rising to
                             min(pending) + delta when none is left.  Changes the order of
                             the relaxations only, never a distance. */
-    int32_t reserved[5];
+    int32_t pack_hold;   /* source-packed batches without a priority window: c in 1..31 holds back,
+                            for one compaction pass, a vertex of out-degree >= 2^(c-1) that fewer
+                            than 3/4 of its pack's sources have reached yet, so that the sources
+                            arriving in the next iteration are expanded with the pending ones in
+                            one read of its edges.  0 = never.  Order only, never a distance. */
+    int32_t reserved[4];
 } np_sched;
 
 

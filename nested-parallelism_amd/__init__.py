@@ -58,12 +58,13 @@ class Sched(ctypes.Structure):
     _fields_ = [("level", ctypes.c_int32), ("log2_t_lo", ctypes.c_int32), ("log2_t_hi", ctypes.c_int32),
                 ("drive", ctypes.c_int32), ("frontier", ctypes.c_int32), ("wgs_per_cu", ctypes.c_int32),
                 ("graph_chunk", ctypes.c_int32), ("max_iters", ctypes.c_int32), ("trace", ctypes.c_int32),
-                ("scheme", ctypes.c_int32), ("delta", ctypes.c_int32), ("reserved", ctypes.c_int32 * 5)]
+                ("scheme", ctypes.c_int32), ("delta", ctypes.c_int32), ("pack_hold", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 4)]
 
     def __init__(self, level=1, log2_t_lo=6, log2_t_hi=6, drive=DRIVE_PERSISTENT, frontier=FRONTIER_SCAN,
-                 wgs_per_cu=0, graph_chunk=0, max_iters=0, trace=0, scheme=0, delta=0):
+                 wgs_per_cu=0, graph_chunk=0, max_iters=0, trace=0, scheme=0, delta=0, pack_hold=0):
         super().__init__(level, log2_t_lo, log2_t_hi, drive, frontier, wgs_per_cu, graph_chunk, max_iters,
-                         trace, scheme, delta)
+                         trace, scheme, delta, pack_hold)
 
     def copy(self, **kw):
         d = {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}

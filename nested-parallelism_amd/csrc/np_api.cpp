@@ -299,6 +299,7 @@ static int validate_sched(const np_sched *in, np_sched *out) {
     if (out->scheme < NP_SCHEME_CONSOLIDATE || out->scheme > NP_SCHEME_BFS_HIER) {
         set_error("scheme must be 0..8"); return NP_ERR_ARG;
     }
+    if (out->pack_hold < 0 || out->pack_hold >= NCLS) { set_error("pack_hold must be 0..%d", NCLS - 1); return NP_ERR_ARG; }
     return NP_OK;
 }
 
@@ -1791,6 +1792,11 @@ int np_batch_solve(np_ctx *c, np_batch *b, const np_sched *sched, np_stats *stat
         st.mflag = s.mflag;
         st.qmask[0] = s.qmask[0]; st.qmask[1] = s.qmask[1];
         st.count_work = sch.trace ? 1 : 0;
+        if (psch.pack_hold > 0 && psch.delta <= 0) {
+            // hold a heavy vertex while fewer than 3/4 of the pack's sources have reached it
+            st.hold_cls = psch.pack_hold;
+            st.hold_min = (3 * (int)pk->inst.size() + 3) / 4;
+        }
         SchedDev sd = make_schedd(c, psch, g->n);
         if (psch.max_iters <= 0) {
             // Without a window every source's frontier empties within n+1 expansions; with
@@ -2297,6 +2303,18 @@ int np_batch_autotune(np_ctx *c, np_batch *b, const np_sched *base, np_sched *be
                     if (st.kernel_ms < bestt) { bestt = st.kernel_ms; bs = s; }
                 }
             }
+    // packs without a window: holding back heavy vertices that few sources have reached yet
+    // (out-degree >= 256, 512, 1024) on top of the best schedule
+    if (!b->packs.empty()) {
+        const np_sched base_bs = bs;
+        for (int hold : {9, 10, 11}) {
+            np_sched s = base_bs;
+            s.pack_hold = hold;
+            np_stats st;
+            if ((rc = np_batch_solve(c, b, &s, &st))) return rc;
+            if (st.kernel_ms < bestt) { bestt = st.kernel_ms; bs = s; }
+        }
+    }
     // priority window on top of the best schedule, scaled by the weights of the graph
     // that carries the most instances
     double wmean = 0;

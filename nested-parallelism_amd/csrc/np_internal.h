@@ -62,6 +62,11 @@ struct DevState {
     // source-packed, host-loop and graph drives: a counting kernel after each compaction
     // accumulates Ctrl::edges_total / items_total (set when the solve records a trace)
     int count_work;
+    // source-packed, no priority window: a flagged vertex of out-degree class >= hold_cls that
+    // fewer than hold_min of the pack's sources have reached is held back one compaction pass
+    // (once per flagging; bm[1] marks the held vertices); hold_cls 0 = never
+    int hold_cls;
+    int hold_min;
 };
 
 // Largest number of sources packed into one distance row.

@@ -668,11 +668,14 @@ __global__ __launch_bounds__(WG) void k_decide(DevState st, SchedDev sp, int it_
     const int pass = ld_coherent(&ctrl->pass);  // the compaction of this slot
     if (tot == 0 && !sp.push) {
         const int T = ld_coherent(&ctrl->thresh);
-        if (T != NO_WINDOW && ld_coherent(ctrl->defer_cnt + pass % 3) > 0) {
-            // nothing below the window: raise it; this slot's expansion is skipped
+        if ((T != NO_WINDOW || st.hold_cls) && ld_coherent(ctrl->defer_cnt + pass % 3) > 0) {
+            // nothing queued but pairs kept back: raise the window (held vertices need only the
+            // next pass); this slot's expansion is skipped
             if (threadIdx.x == 0) {
-                const int mn = ld_coherent(ctrl->defer_min + pass % 3);
-                ctrl->thresh = mn > NO_WINDOW - sp.delta ? NO_WINDOW : mn + sp.delta;
+                if (T != NO_WINDOW) {
+                    const int mn = ld_coherent(ctrl->defer_min + pass % 3);
+                    ctrl->thresh = mn > NO_WINDOW - sp.delta ? NO_WINDOW : mn + sp.delta;
+                }
                 ctrl->skip = 1;
                 ctrl->pass = pass + 1;
             }
@@ -1286,6 +1289,23 @@ __device__ void pk_expansion(const DevState &st, const int *q, const unsigned *q
     }
 }
 
+// Number of the pack's sources that have reached v (finite entries of its distance row; the
+// row's unused entries stay at inf).
+__device__ __forceinline__ int pk_reached(const DevState &st, long long v) {
+    const int *row = st.dist + (v << st.kshift);
+    int r = 0;
+    if (st.kshift < 2) {
+        for (int i = 0; i < (1 << st.kshift); ++i) r += ld_coherent(row + i) < st.inf;
+        return r;
+    }
+    const int4 *row4 = (const int4 *)row;
+    for (int q = 0; q < (1 << (st.kshift - 2)); ++q) {
+        const int4 x = row4[q];
+        r += (x.x < st.inf) + (x.y < st.inf) + (x.z < st.inf) + (x.w < st.inf);
+    }
+    return r;
+}
+
 // The pending sources of v (mask m) whose distance at v is below the window T; *dmin
 // receives the least distance among the others.  The row is read with 16-byte loads
 // through the caches: a value older than the last atomicMin is only larger, so it can
@@ -1342,11 +1362,20 @@ __device__ __forceinline__ void pk_compaction(const DevState &st, int *cnt_out, 
          wb += (long long)gridDim.x * WG) {
         const long long wi = wb + threadIdx.x;
         const unsigned word = wi < st.nwords ? ld_coherent_u(bm + wi) : 0u;
-        unsigned queued = 0;
+        unsigned queued = 0, held = 0, hw = 0;
+        bool hw_read = false;
         for (unsigned w = word; w; w &= w - 1) {
             const int b = __ffs(w) - 1;
             const int c = st.cls[wi * 32 + b];
             if (!c) continue;  // no out-edges: dropped (its mflag stays set, so it is not flagged again)
+            if (st.hold_cls && c >= st.hold_cls) {
+                if (!hw_read) { hw = ld_coherent_u(st.bm[1] + wi); hw_read = true; }
+                if (!((hw >> b) & 1u) && pk_reached(st, wi * 32 + b) < st.hold_min) {
+                    held |= 1u << b;  // more sources are on their way: wait one pass, keep the flag
+                    my_dc++;
+                    continue;
+                }
+            }
             queued |= 1u << b;
             atomicAdd(&s_lc[c], 1);
         }
@@ -1363,7 +1392,10 @@ __device__ __forceinline__ void pk_compaction(const DevState &st, int *cnt_out, 
             }
             pairs += __popc(t);
         }
-        if (word) st_coherent_u(bm + wi, 0u);
+        if (word) st_coherent_u(bm + wi, held);
+        // a held vertex stays flagged, so the held bits are always a subset of the flags: they
+        // become exactly this pass's held vertices (the ones held before are queued now)
+        if (hw_read && hw != held) st_coherent_u(st.bm[1] + wi, held);
         __syncthreads();
         if (threadIdx.x < NCLS) s_lc[threadIdx.x] = 0;  // the second loop counted in s_lc
         __syncthreads();
@@ -1418,7 +1450,7 @@ __device__ __forceinline__ void pk_compaction(const DevState &st, int *cnt_out, 
     for (int o = 32; o > 0; o >>= 1) pairs += __shfl_xor(pairs, o);
     if (lane_id() == 0 && pairs) atomicAdd((unsigned long long *)&st.ctrl->pairs_total, (unsigned long long)pairs);
 
-    if (T != NO_WINDOW) flush_deferred(my_dc, my_dm, &s_dc, &s_dm, defer_cnt, defer_min);
+    if (T != NO_WINDOW || st.hold_cls) flush_deferred(my_dc, my_dm, &s_dc, &s_dm, defer_cnt, defer_min);
 }
 
 // Lane-bin bound of a packed expansion (see PACK_LANE_CLS); the other bins keep theirs.
@@ -1470,11 +1502,15 @@ __global__ __launch_bounds__(WG) void k_pk_persistent(DevState st, SchedDev sp) 
                 s_total = tot > 0x7FFFFFFF ? 0x7FFFFFFF : (int)tot;
             }
             __syncthreads();
-            if (s_total > 0 || T == NO_WINDOW) break;
-            // nothing below the window: raise it to the least pending distance + delta
+            if (s_total > 0) break;
+            // nothing queued: done unless pairs were kept back (below the window, or held
+            // vertices, which the next pass takes); a window rises to the least pending
+            // distance + delta
             if (ld_coherent(dc) == 0) break;
-            const int mn = ld_coherent(dm);
-            T = mn > NO_WINDOW - sp.delta ? NO_WINDOW : mn + sp.delta;
+            if (T != NO_WINDOW) {
+                const int mn = ld_coherent(dm);
+                T = mn > NO_WINDOW - sp.delta ? NO_WINDOW : mn + sp.delta;
+            }
         }
         const int total = s_total;
         if (total == 0 || (sp.max_iters > 0 && it >= sp.max_iters)) {
@@ -2009,6 +2045,7 @@ hipError_t launch_init_packed(const DevState &st, const int *d_srcs, int nsrc, i
     hipError_t e;
     if ((e = launch_fill_i32(st.dist, st.nslots << st.kshift, st.inf, s)) != hipSuccess) return e;
     if ((e = launch_fill_u32(st.bm[0], st.nwords, 0u, s)) != hipSuccess) return e;
+    if (st.hold_cls && (e = launch_fill_u32(st.bm[1], st.nwords, 0u, s)) != hipSuccess) return e;
     if ((e = launch_fill_u32(st.mflag, st.nslots, 0u, s)) != hipSuccess) return e;
     if ((e = launch_fill_i32(st.cnt, 3 * NCLS, 0, s)) != hipSuccess) return e;
     if (st.sbcnt && (e = launch_fill_i32(st.sbcnt, 3, 0, s)) != hipSuccess) return e;

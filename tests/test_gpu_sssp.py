@@ -480,6 +480,35 @@ def test_bfs_device_matches_bfs(npa, ctx, orc):
         G.free()
 
 
+@pytest.mark.parametrize("drive", [0, 1, 2])
+def test_packed_hold(npa, ctx, orc, drive):
+    """np_sched.pack_hold keeps heavy vertices that few of the pack's sources have reached back
+    for one pass: distances unchanged for every hold class -- class 1 makes every flagged vertex a
+    candidate, so whole passes queue nothing and the drives must run the next pass instead of
+    stopping -- for SSSP and BFS packs, with and without a window (a window turns it off)."""
+    G = npa.Graph.rmat(ctx, 14, 16, wmin=1, wmax=255, seed=95)
+    row, col, w = G.download()
+    srcs = [int(s) for s in npa.pick_sources(row, 16, 95)]
+    refs = [orc.dijkstra(row, col, w, s)[0] for s in srcs]
+    lrefs = [orc.bfs_levels(row, col, s) for s in srcs]
+    B = npa.Batch(ctx, [G] * len(srcs), srcs)
+    Bu = npa.Batch(ctx, [G] * len(srcs), srcs, unit=True)
+    for hold in (0, 1, 4, 8, 10):
+        for delta in (0, 300):
+            st = B.solve(_sched(npa, 2, 3, 10, drive, 0, delta=delta, pack_hold=hold))
+            assert st.iterations > 0
+            for i, ref in enumerate(refs):
+                assert np.array_equal(B.get(i), ref), (hold, delta, i)
+        Bu.solve(_sched(npa, 2, 3, 10, drive, 0, pack_hold=hold))
+        for i, ref in enumerate(lrefs):
+            assert np.array_equal(Bu.get(i), ref), ("bfs", hold, i)
+    with pytest.raises(npa.NpError):
+        B.solve(_sched(npa, 2, 3, 10, drive, 0, pack_hold=-1))
+    B.free()
+    Bu.free()
+    G.free()
+
+
 def test_pack_max_option(npa, ctx, orc):
     """NP_CTX_PACK_MAX caps the sources per pack: 20 sources on one graph go to ceil(20 / max)
     packs of near-equal size, each solved exactly; values outside 2..32 are refused."""
