@@ -488,7 +488,7 @@ def bfs_bench(npa, ctx, args):
     vertex at every level), 1 (naive recursion: a workgroup per frontier vertex, children pushed
     by the relaxers) and 2 (hierarchical: two levels per expansion) on the host loop as the
     reference runs them; solution 3 (consolidation) at levels 0/1/2 and level 2 on the graph
-    drive; and the 16 sources source-packed in one batch.  GTEPS = sum over sources of the
+    drive; and the 16 sources source-packed in one batch on the batch autotuner's schedule.  GTEPS = sum over sources of the
     out-edges of reached vertices / step time.  Algorithmic bytes per source = 8 per reached
     edge (col 4 + the level gather 4; no weights) + 16 per reached vertex (row bounds 8, own
     level 4, final level write 4).  Every level array of every mode is compared with the
@@ -520,6 +520,18 @@ def bfs_bench(npa, ctx, args):
     modes = [m for m in (args.modes.split(",") if args.modes else scheds) if m in scheds]
     d_lev = npa.DeviceBuffer(ctx, G.n)
     B = npa.Batch(ctx, [G] * len(srcs), srcs, unit=True) if "batched" in modes else None
+    if B is not None:
+        fixed = [x for x in args.sched if x.startswith("batched=")]
+        if fixed:
+            v = [int(x) for x in fixed[0].split("=")[1].split(",")]
+            scheds["batched"] = npa.Sched(level=v[0], log2_t_lo=v[1], log2_t_hi=v[2], wgs_per_cu=v[3],
+                                          drive=v[4] if len(v) > 4 else npa.DRIVE_PERSISTENT,
+                                          pack_hold=v[6] if len(v) > 6 else 0)
+        else:
+            t = time.time()
+            bs, bms = B.autotune(npa.Sched(level=1))
+            scheds["batched"] = bs.copy(trace=0)
+            log("batch autotune: %r (%.3f ms/solve, %.1fs)" % (bs, bms, time.time() - t))
     results, finals = {}, {}
     for mode in modes:
         sch = scheds[mode]
