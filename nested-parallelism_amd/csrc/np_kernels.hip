@@ -1356,8 +1356,9 @@ __device__ __forceinline__ void pk_compaction(const DevState &st, int *cnt_out, 
     __syncthreads();
     int pairs = 0, my_dc = 0, my_dm = NO_WINDOW;
     const long long nv = st.nwords * 32;
-    // Without a window every pending vertex is taken whole and nothing stays flagged: the frontier
-    // is as sparse as a single source's, so a thread scans one bitmap word as compaction_impl does.
+    // Without a window every pending vertex is taken whole and nothing stays flagged but the hubs
+    // held for one pass (hold_cls): the frontier is as sparse as a single source's, so a thread
+    // scans one bitmap word as compaction_impl does.
     for (long long wb = (long long)blockIdx.x * WG; T == NO_WINDOW && wb < st.nwords;
          wb += (long long)gridDim.x * WG) {
         const long long wi = wb + threadIdx.x;
