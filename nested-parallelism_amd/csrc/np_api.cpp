@@ -2304,10 +2304,10 @@ int np_batch_autotune(np_ctx *c, np_batch *b, const np_sched *base, np_sched *be
                 }
             }
     // packs without a window: holding back heavy vertices that few sources have reached yet
-    // (out-degree >= 256, 512, 1024) on top of the best schedule
+    // (out-degree >= 64 .. 2048) on top of the best schedule
     if (!b->packs.empty()) {
         const np_sched base_bs = bs;
-        for (int hold : {9, 10, 11}) {
+        for (int hold : {7, 8, 9, 10, 11, 12}) {
             np_sched s = base_bs;
             s.pack_hold = hold;
             np_stats st;
@@ -2329,6 +2329,7 @@ int np_batch_autotune(np_ctx *c, np_batch *b, const np_sched *base, np_sched *be
         };
         if ((rc = tune_window(eval, wmean, &bs, &bestt))) return rc;
     }
+    if (bs.delta > 0) bs.pack_hold = 0;  // a window turns the hold off: report what runs
     *best = bs;
     if (best_ms) *best_ms = bestt;
     return NP_OK;
