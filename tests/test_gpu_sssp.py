@@ -394,6 +394,12 @@ def test_packed_batch(npa, ctx, orc):
             assert st.iterations > 0
             for i in range(len(graphs)):
                 assert np.array_equal(B.get(i), refs[i]), (drive, lv, lo, hi, i)
+        # the hub hold on packs of 20 (rows of 32 with 12 unused entries) and 3 sources, a
+        # duplicate and a degree-0 source among them, beside the concatenated part it leaves alone
+        for hold in (1, 5):
+            B.solve(_sched(npa, 2, 2, 4, drive, 0, pack_hold=hold))
+            for i in range(len(graphs)):
+                assert np.array_equal(B.get(i), refs[i]), ("hold", hold, drive, i)
     for i in (0, 39, 41, len(graphs) - 1):
         assert np.array_equal(_dev_to_host(npa, ctx, B.device_dist(i), graphs[i].n), refs[i]), i
     B.free()
