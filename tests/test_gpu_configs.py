@@ -6,7 +6,8 @@ oracle's Dijkstra, bit-exact.
   C2  R-MAT s22 (4,194,304 vertices, 67,108,864 edges), 8 sources (seed 2): block, warp and
       grid consolidation per source, and the source-packed batch
   C3  R-MAT s24 (16,777,216 / 268,435,456), 16 sources (seed 3): the plain fixed-threshold
-      schedule, the learned scheduler, the source-packed batch and the learned batch
+      schedule, the learned scheduler, the source-packed batch (with and without the hub hold)
+      and the learned batch; BFS levels of the same sources, alone and source-packed
   C4  R-MAT s20 x 64 sources (seed 4) + 256 R-MAT graphs of scale U{12..18} (seeds
       1000-1255): the concatenated launch and the packed + concatenated batch
   C5  4096 x 4096 grid + 1% long-range edges (seed 5), 32 sources: the batch with a priority
@@ -118,12 +119,35 @@ def test_c3_rmat_s24_batched(npa, ctx, c3, learned):
     assert B.layout() == (1, 0)
     lv = npa.LEVEL_LEARNED if learned else 2
     for sch in (npa.Sched(level=lv, log2_t_lo=6, log2_t_hi=12, drive=npa.DRIVE_GRAPH),
-                npa.Sched(level=lv, log2_t_lo=5, log2_t_hi=10, drive=npa.DRIVE_PERSISTENT)):
+                npa.Sched(level=lv, log2_t_lo=5, log2_t_hi=10, drive=npa.DRIVE_PERSISTENT),
+                npa.Sched(level=lv, log2_t_lo=4, log2_t_hi=10, drive=npa.DRIVE_GRAPH, pack_hold=11),
+                npa.Sched(level=lv, log2_t_lo=4, log2_t_hi=10, drive=npa.DRIVE_PERSISTENT, pack_hold=10)):
         st = B.solve(sch)
         assert st.iterations > 0
         for i, ref in enumerate(c3.plain):
             assert np.array_equal(B.get(i), ref), ("C3 batched", learned, sch, i)
     B.free()
+
+
+@pytest.mark.timeout(300)
+def test_c3_rmat_s24_bfs_levels(npa, ctx, orc, c3):
+    """BFS levels (BFS_REC_GPU's result) on the C3 graph against the oracle's queue BFS: one
+    source alone at block level, and the 16 sources source-packed with and without the hub hold."""
+    from concurrent.futures import ThreadPoolExecutor
+    row, col, _ = c3.host
+    with ThreadPoolExecutor(max(1, min(16, os.cpu_count() or 1))) as ex:
+        refs = list(ex.map(lambda s: orc.bfs_levels(row, col, s), c3.srcs))
+    lev, _ = ctx.bfs(c3.G, c3.srcs[0], npa.Sched(level=1))
+    assert np.array_equal(lev, refs[0])
+    Bu = npa.Batch(ctx, [c3.G] * len(c3.srcs), c3.srcs, unit=True)
+    assert Bu.layout() == (1, 0)
+    for sch in (npa.Sched(level=2, log2_t_lo=4, log2_t_hi=12, drive=npa.DRIVE_GRAPH, pack_hold=7),
+                npa.Sched(level=2, log2_t_lo=4, log2_t_hi=12, drive=npa.DRIVE_PERSISTENT)):
+        st = Bu.solve(sch)
+        assert st.iterations > 0
+        for i, ref in enumerate(refs):
+            assert np.array_equal(Bu.get(i), ref), ("C3 BFS packed", sch, i)
+    Bu.free()
 
 
 # --------------------------------------------------------------------------- C4
