@@ -68,7 +68,8 @@ def test_c2_rmat_s22_levels_and_packed(npa, ctx, orc):
     B = npa.Batch(ctx, [G] * len(srcs), srcs)
     assert B.layout() == (1, 0)
     for sch in (npa.Sched(level=2, log2_t_lo=5, log2_t_hi=12, drive=npa.DRIVE_GRAPH),
-                npa.Sched(level=1, drive=npa.DRIVE_PERSISTENT)):
+                npa.Sched(level=1, drive=npa.DRIVE_PERSISTENT),
+                npa.Sched(level=2, log2_t_lo=4, log2_t_hi=8, drive=npa.DRIVE_GRAPH, pack_hold=10)):
         B.solve(sch)
         for i in range(len(srcs)):
             assert np.array_equal(B.get(i), plain[i]), ("C2 packed", sch, i)
@@ -210,8 +211,10 @@ def test_c5_grid_batched_learned_and_window(npa, ctx, orc):
     ref, _ = orc.dijkstra(*host, srcs[0])
     assert np.array_equal(first[0], ref)
     _load_model(npa, ctx)
+    # a window turns the hub hold off: the pack_hold of the last schedule changes nothing
     for sch in (npa.Sched(level=npa.LEVEL_LEARNED, log2_t_lo=6, log2_t_hi=12, drive=npa.DRIVE_GRAPH, delta=4004),
-                npa.Sched(level=npa.LEVEL_LEARNED, drive=npa.DRIVE_PERSISTENT)):
+                npa.Sched(level=npa.LEVEL_LEARNED, drive=npa.DRIVE_PERSISTENT),
+                npa.Sched(level=2, log2_t_lo=4, log2_t_hi=14, drive=npa.DRIVE_GRAPH, delta=4004, pack_hold=9)):
         B.solve(sch)
         for i in range(len(srcs)):
             assert np.array_equal(B.get(i), first[i]), ("C5 learned batch", sch, i)
