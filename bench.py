@@ -625,7 +625,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only; --gpus %d is not supported "
-              "(instances shard across GPUs with no communication; see DESIGN.md)" % args.gpus, file=sys.stderr)
+              "(a batch's instances would be split across GPUs, one process each, with no communication)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     npa = load_pkg()

@@ -321,7 +321,7 @@ int np_model_get(np_ctx *ctx, float *params);
  * vertices (the same function the kernels evaluate, for host-side training sets). */
 int np_features(const int32_t *count, int32_t it, int32_t n, float *x);
 /* Fixed-threshold heuristic applied when the network is absent or its output is not
- * finite: the fallback and the judge of the learned scheduler. */
+ * finite: the learned scheduler's fallback, and the schedule it is measured against. */
 int np_heuristic(const int32_t *count, int32_t it, int32_t n, np_sched *s);
 /* One full-batch training run on the GPU with the project's forward/backward kernels
  * (Adam).  x[batch][10], label[batch] in {0,1,2}, target[batch][2] (log2 thresholds).

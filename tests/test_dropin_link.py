@@ -1,8 +1,8 @@
 """The drop-in boundary without a source change: the reference's own GPU drivers
 (main() of apps/<app>/src/gpu/<app>.cpp, unchanged, with the reference's headers
 common/global.h and apps/<app>/src/gpu/<app>.h) compile and link against libnpdropin.so
-instead of the reference's CUDA wrappers, with no undefined symbol.  This is the recipe of
-INTEGRATION.md section 1, run as written.  Skipped where the reference sources are absent."""
+instead of the reference's CUDA wrappers, with no undefined symbol: the documented link recipe,
+run as written.  Skipped where the reference sources are absent."""
 import os
 import subprocess
 

@@ -561,7 +561,7 @@ def test_learned_scheduler_beats_fixed_heuristic(npa, ctx, orc):
         t_learned += float(np.median([ctx.sssp_device(G, s, d_dist.ptr, learned).kernel_ms for _ in range(5)]))
     d_dist.free()
     G.free()
-    assert t_plain / t_learned >= 1.3, (t_plain, t_learned)   # the north star's bar for the learned scheduler
+    assert t_plain / t_learned >= 1.3, (t_plain, t_learned)   # the learned scheduler's required gain
 
 
 def test_learned_scheduler_not_worse_on_grids(npa, ctx, orc):
