@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <string>
 #include <functional>
+#include <utility>
 #include <mutex>
 
 #include "np_internal.h"
@@ -2284,6 +2285,35 @@ int np_batch_autotune(np_ctx *c, np_batch *b, const np_sched *base, np_sched *be
     if (occ > 4) wgs_opts.push_back(4);
     const int klos[] = {3, 4, 5, 6, 7};
     const int khis[] = {6, 8, 10, 12, 14, 30};
+    // One solve per candidate in the searches; a single timing varies by a few percent, so the
+    // closest candidates are timed twice more and the least of their three timings decides.
+    std::vector<std::pair<double, np_sched>> tried;  // single timings
+    auto once = [&](const np_sched &s, double *ms) {
+        np_stats st;
+        if (int e = np_batch_solve(c, b, &s, &st)) return e;
+        *ms = st.kernel_ms;
+        tried.emplace_back(st.kernel_ms, s);
+        return (int)NP_OK;
+    };
+    auto confirm = [&](size_t top, np_sched *win, double *wint) {
+        std::stable_sort(tried.begin(), tried.end(),
+                         [](const std::pair<double, np_sched> &x, const std::pair<double, np_sched> &y) {
+                             return x.first < y.first;
+                         });
+        *wint = 1e30;
+        for (size_t i = 0; i < tried.size() && i < top; ++i) {
+            double m = tried[i].first;
+            for (int r = 0; r < 2; ++r) {
+                np_stats st;
+                if (int e = np_batch_solve(c, b, &tried[i].second, &st)) return e;
+                m = std::min(m, (double)st.kernel_ms);
+            }
+            if (m < *wint) { *wint = m; *win = tried[i].second; }
+        }
+        tried.clear();
+        tried.emplace_back(*wint, *win);  // confirmed: competes with the next stage's timings
+        return (int)NP_OK;
+    };
     double bestt = 1e30;
     np_sched bs = b0;
     for (int drive : {(int)NP_DRIVE_PERSISTENT, (int)NP_DRIVE_GRAPH})
@@ -2298,11 +2328,11 @@ int np_batch_autotune(np_ctx *c, np_batch *b, const np_sched *base, np_sched *be
                     s.drive = drive;
                     s.level = lv; s.log2_t_lo = klo; s.log2_t_hi = khi; s.wgs_per_cu = wg;
                     s.trace = 0;
-                    np_stats st;
-                    if ((rc = np_batch_solve(c, b, &s, &st))) return rc;
-                    if (st.kernel_ms < bestt) { bestt = st.kernel_ms; bs = s; }
+                    double t;
+                    if ((rc = once(s, &t))) return rc;
                 }
             }
+    if ((rc = confirm(4, &bs, &bestt))) return rc;
     // packs without a window: holding back heavy vertices that few sources have reached yet
     // (out-degree >= 64 .. 2048) on top of the best schedule
     if (!b->packs.empty()) {
@@ -2310,10 +2340,10 @@ int np_batch_autotune(np_ctx *c, np_batch *b, const np_sched *base, np_sched *be
         for (int hold : {7, 8, 9, 10, 11, 12}) {
             np_sched s = base_bs;
             s.pack_hold = hold;
-            np_stats st;
-            if ((rc = np_batch_solve(c, b, &s, &st))) return rc;
-            if (st.kernel_ms < bestt) { bestt = st.kernel_ms; bs = s; }
+            double t;
+            if ((rc = once(s, &t))) return rc;
         }
+        if ((rc = confirm(3, &bs, &bestt))) return rc;
     }
     // priority window on top of the best schedule, scaled by the weights of the graph
     // that carries the most instances
@@ -2321,13 +2351,8 @@ int np_batch_autotune(np_ctx *c, np_batch *b, const np_sched *base, np_sched *be
     const np_graph *g0 = b->packs.empty() ? b->graphs[0] : b->packs[0]->g;
     if ((rc = sample_mean_weight(c, g0, &wmean))) return rc;
     if (!b->unit) {
-        auto eval = [&](const np_sched &s, double *ms) {
-            np_stats st;
-            if (int e = np_batch_solve(c, b, &s, &st)) return e;
-            *ms = st.kernel_ms;
-            return (int)NP_OK;
-        };
-        if ((rc = tune_window(eval, wmean, &bs, &bestt))) return rc;
+        if ((rc = tune_window(once, wmean, &bs, &bestt))) return rc;
+        if ((rc = confirm(3, &bs, &bestt))) return rc;
     }
     if (bs.delta > 0) bs.pack_hold = 0;  // a window turns the hold off: report what runs
     *best = bs;
