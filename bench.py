@@ -431,6 +431,158 @@ def measured_traffic(config):
         return None
 
 
+def sched_arg(s):
+    """The --sched value of schedule s: level,log2_t_lo,log2_t_hi,wgs_per_cu,drive,delta,pack_hold."""
+    return ",".join(str(int(x)) for x in (s.level, s.log2_t_lo, s.log2_t_hi, s.wgs_per_cu, s.drive, s.delta,
+                                             s.pack_hold))
+
+
+def traffic_child(npa, ctx, args):
+    """--traffic-child: the solve whose HBM traffic the parent measures, run under rocprofv3 --pmc.
+    The workload and batch of --config, one warm-up solve of the --sched solve=... schedule, then
+    a reached-work count, the measured solve, a second count, and three 1 GiB device-to-device
+    copies.  The two k_reached_work dispatches delimit the solve's dispatches in the counter file;
+    the copies scale its counters."""
+    unit = args.config == "bfs"
+    W = build_workload(npa, ctx, "c3" if unit else args.config, args)
+    v = [int(x) for x in [x for x in args.sched if x.startswith("solve=")][0].split("=")[1].split(",")]
+    sch = npa.Sched(level=v[0], log2_t_lo=v[1], log2_t_hi=v[2], wgs_per_cu=v[3], drive=v[4], delta=v[5],
+                    pack_hold=v[6])
+    if sch.level == npa.LEVEL_LEARNED:
+        ctx.model_set(np.fromfile(os.path.join(PKG_DIR, "sched_mlp.f32"), dtype=np.float32))
+    ctx.set_option(npa.CTX_PACK, 1)
+    B = npa.Batch(ctx, [W.graphs[gi] for gi, _ in W.inst], [s for _, s in W.inst], unit=unit)
+    G = W.graphs[0]
+    d = npa.DeviceBuffer(ctx, G.n)
+    B.solve(sch)
+    ctx.reached_work(G, d.ptr)
+    st = B.solve(sch)
+    ctx.reached_work(G, d.ptr)
+    copy_calibration(npa, ctx, 1 << 30)
+    print(json.dumps({"traffic_child": True, "drive": int(st.drive), "requested_drive": int(sch.drive),
+                      "kernel_ms": float(st.kernel_ms)}), flush=True)
+    d.free()
+    B.free()
+    for g in W.graphs:
+        g.free()
+    return 0
+
+
+def _counter_pass(prof, counter, child_args, tmp):
+    """One rocprofv3 --pmc pass over a child bench.py: (solve KiB, copy KiB, dispatches, child line)
+    or a string saying why there is no figure."""
+    import glob
+    import signal
+    import subprocess
+    d = os.path.join(tmp, counter)
+    cmd = [prof, "--pmc", counter, "-d", d, "-o", "run", "--output-format", "csv", "--",
+           sys.executable, "-u", os.path.abspath(__file__)] + child_args
+    with open(os.path.join(tmp, counter + ".out"), "w") as fo, open(os.path.join(tmp, counter + ".err"), "w") as fe:
+        p = subprocess.Popen(cmd, stdout=fo, stderr=fe, start_new_session=True)
+        try:
+            p.wait(timeout=300)
+        except subprocess.TimeoutExpired:
+            os.killpg(p.pid, signal.SIGKILL)
+            p.wait()
+            return "the %s pass exceeded 300 s and was stopped" % counter
+    child = None
+    for ln in open(os.path.join(tmp, counter + ".out")):
+        if ln.startswith("{") and "traffic_child" in ln:
+            child = json.loads(ln)
+    if child is None:
+        return "the %s pass ended (status %d) before its solve finished" % (counter, p.returncode)
+    files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+    if not files:
+        return "the %s pass (status %d) wrote no counter file" % (counter, p.returncode)
+    r = solve_counter(files)
+    if isinstance(r, str):
+        return "the %s pass: %s" % (counter, r)
+    return r + (child,)
+
+
+def solve_counter(files):
+    """From rocprofv3 counter-collection CSV files of a --traffic-child run: (the counter summed
+    over the dispatches between the last two k_reached_work dispatches, its median over the last
+    three copy kernels, the number of dispatches summed), or a string saying what is missing.
+    A dispatch's rows (one per counter instance) are summed."""
+    import csv
+    import statistics
+    disp = {}
+    for f in files:
+        with open(f) as fh:
+            for r in csv.DictReader(fh):
+                e = disp.setdefault((f, r["Dispatch_Id"]), {"name": r["Kernel_Name"], "t": int(r["Start_Timestamp"]),
+                                                            "v": 0.0})
+                e["v"] += float(r["Counter_Value"])
+    rows = sorted(disp.values(), key=lambda e: e["t"])
+    marks = [i for i, e in enumerate(rows) if "k_reached_work" in e["name"]]
+    copies = [i for i, e in enumerate(rows) if "copyBuffer" in e["name"]]
+    if len(marks) < 2 or len([i for i in copies if i > marks[-1]]) < 3:
+        return "no pair of markers followed by three copies"
+    window = [e for e in rows[marks[-2] + 1:marks[-1]] if "copyBuffer" not in e["name"]]
+    if not window:
+        return "no dispatch between the markers"
+    return sum(e["v"] for e in window), statistics.median(rows[i]["v"] for i in copies[-3:]), len(window)
+
+
+def live_traffic(args, config, sch):
+    """HBM bytes of one solve of schedule sch, measured in this run: two child processes build the
+    same workload and run the solve under rocprofv3 --pmc, one for FETCH_SIZE and one for
+    WRITE_SIZE.  Each counter is scaled by what it reads for a 1 GiB device-to-device copy in the
+    same child (a copy reads and writes exactly 1 GiB).  Returns (bytes, note) or (None, reason)."""
+    import shutil
+    import tempfile
+    if any(k.startswith("ROCPROF") for k in os.environ):
+        return None, "bench.py itself runs under rocprofv3"
+    prof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(prof):
+        return None, "rocprofv3 not found"
+    tmp = tempfile.mkdtemp(prefix="np_traffic_")
+    child_args = ["--config", config, "--traffic-child", "--scale", str(args.scale), "--sources", str(args.sources),
+                  "--sched", "solve=" + sched_arg(sch)]
+    got = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        t = time.time()
+        r = _counter_pass(prof, counter, child_args, tmp)
+        if isinstance(r, str):
+            log("traffic: %s" % r)
+            return None, r
+        kib, copy_kib, nd, child = r
+        if child["drive"] != child["requested_drive"]:
+            return None, "the profiled solve ran on drive %d, not %d" % (child["drive"], child["requested_drive"])
+        got[counter] = (kib, copy_kib, nd)
+        log("traffic: %s %.0f KiB over %d dispatches of one solve; 1 GiB copy %.0f KiB (%.1fs)" % (
+            counter, kib, nd, copy_kib, time.time() - t))
+    shutil.rmtree(tmp, ignore_errors=True)
+    (fk, fc, nd), (wk, wc, _) = got["FETCH_SIZE"], got["WRITE_SIZE"]
+    fscale, wscale = 1048576.0 / fc, 1048576.0 / wc
+    traffic = 1024.0 * (fk * fscale + wk * wscale)
+    return int(traffic), {
+        "source": "measured in this run: rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, each over one solve of this "
+                  "schedule in a child process, scaled by the counter's reading for a 1 GiB device copy",
+        "unscaled_bytes": int(1024.0 * (fk + wk)), "fetch_kib": round(fk), "write_kib": round(wk),
+        "fetch_scale": round(fscale, 4), "write_scale": round(wscale, 4), "dispatches": nd}
+
+
+def roofline_traffic(args, config, head, sch, batched):
+    """roofline.traffic of the line: measured in this run (--traffic live, for a batch solve), else
+    the stored record of profiles/traffic_<config>.json, marked as such."""
+    reason = None
+    if args.traffic == "live" and batched:
+        traffic, note = live_traffic(args, config, sch)
+        if traffic is not None:
+            stored, _ = stored_traffic(config, head)
+            note["stored_bytes"] = stored
+            return traffic, note
+        reason = note
+    if args.traffic == "off":
+        return None, None
+    traffic, note = stored_traffic(config, head)
+    if note is not None and reason:
+        note["live_measurement"] = reason
+    return traffic, note
+
+
 def spmv_bench(npa, ctx, args):
     """--config spmv: SpMV_GPU on R-MAT scale 22 (4,194,304 rows, 67,108,864 entries, ids permuted)
     with the reference's values (data[i] = i % 13 + 0.5, x[i] = i % 17 + 0.5, float64: every sum is
@@ -575,7 +727,7 @@ def bfs_bench(npa, ctx, args):
     hr = results[head]
     per_launch_bytes = float(b_alg.sum()) if head == "batched" else float(b_alg.mean())
     achieved = per_launch_bytes / (hr["launch_ms"] * 1e-3) / 1e9
-    traffic, traffic_note = stored_traffic("bfs", head)
+    traffic, traffic_note = roofline_traffic(args, "bfs", head, scheds[head], head == "batched")
     drive = scheds[head].drive
     kernel = ("k_pk_compact+k_decide+k_pk_expand" if head == "batched" else
               {"sol0_flat": "k_bfs_flat", "sol2_hier": "k_bfs_hier"}.get(head, "k_persistent" if drive == 0 else
@@ -622,6 +774,11 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="after the run, copy this many bytes device-to-device 3 times (a copy kernel of known "
                          "size for scaling FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--traffic", choices=("live", "stored", "off"), default="live",
+                    help="roofline.traffic of a batched line: live = rocprofv3 --pmc passes over one solve in child "
+                         "processes during this run (about a minute), stored = the profiles/traffic_<config>.json "
+                         "record, off = null")
+    ap.add_argument("--traffic-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only; --gpus %d is not supported "
@@ -631,7 +788,7 @@ def main():
     npa = load_pkg()
     ctx = npa.Context(0)
     try:
-        rc = run_config(npa, ctx, args)
+        rc = traffic_child(npa, ctx, args) if args.traffic_child else run_config(npa, ctx, args)
     finally:
         ctx.close()  # every device object is released before the interpreter exits
     if rc:
@@ -781,7 +938,7 @@ def run_config(npa, ctx, args):
         per_launch_bytes = float(b_alg.mean())
     kernel += " (%s drive)" % {0: "persistent", 1: "host-loop", 2: "graph"}[scheds[head].drive]
     achieved = per_launch_bytes / (hr["launch_ms"] * 1e-3) / 1e9
-    traffic, traffic_note = stored_traffic(args.config, head)
+    traffic, traffic_note = roofline_traffic(args, args.config, head, scheds[head], head in batches)
     # work of one traced solve of the fastest mode (not timed): edge rows read (out-degrees of the queued
     # vertices), (edge, source) items and queued (vertex, source) pairs; for packs the mean number of
     # sources pending per edge row read and the re-expansion of pairs over the reached ones
