@@ -541,19 +541,21 @@ def live_traffic(args, config, sch):
     child_args = ["--config", config, "--traffic-child", "--scale", str(args.scale), "--sources", str(args.sources),
                   "--sched", "solve=" + sched_arg(sch)]
     got = {}
-    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
-        t = time.time()
-        r = _counter_pass(prof, counter, child_args, tmp)
-        if isinstance(r, str):
-            log("traffic: %s" % r)
-            return None, r
-        kib, copy_kib, nd, child = r
-        if child["drive"] != child["requested_drive"]:
-            return None, "the profiled solve ran on drive %d, not %d" % (child["drive"], child["requested_drive"])
-        got[counter] = (kib, copy_kib, nd)
-        log("traffic: %s %.0f KiB over %d dispatches of one solve; 1 GiB copy %.0f KiB (%.1fs)" % (
-            counter, kib, nd, copy_kib, time.time() - t))
-    shutil.rmtree(tmp, ignore_errors=True)
+    try:
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            t = time.time()
+            r = _counter_pass(prof, counter, child_args, tmp)
+            if isinstance(r, str):
+                log("traffic: %s" % r)
+                return None, r
+            kib, copy_kib, nd, child = r
+            if child["drive"] != child["requested_drive"]:
+                return None, "the profiled solve ran on drive %d, not %d" % (child["drive"], child["requested_drive"])
+            got[counter] = (kib, copy_kib, nd)
+            log("traffic: %s %.0f KiB over %d dispatches of one solve; 1 GiB copy %.0f KiB (%.1fs)" % (
+                counter, kib, nd, copy_kib, time.time() - t))
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
     (fk, fc, nd), (wk, wc, _) = got["FETCH_SIZE"], got["WRITE_SIZE"]
     fscale, wscale = 1048576.0 / fc, 1048576.0 / wc
     traffic = 1024.0 * (fk * fscale + wk * wscale)
