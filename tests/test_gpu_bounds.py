@@ -3,8 +3,10 @@ source-mask and delayed-buffer write is checked against its segment; a violation
 fails the solve with NP_ERR_BOUNDS.  One child process runs the solves that fill those buffers
 -- every drive and frontier mode, the consolidation levels, the schemes of the reference's
 solutions (delayed buffers included), the source-packed batch with and without a priority
-window, the concatenated batch and BFS -- and checks every result against the oracle, so a
-passing run means no write left its segment and the checks change no result."""
+window and with the hub hold, the concatenated batch and BFS -- and checks every result against
+the oracle, so a passing run means no write left its segment and the checks change no result.
+A second child does the same at full size: the R-MAT s24 graph and 16 sources of the headline
+benchmark, source-packed with the hub hold on the graph and persistent drives, SSSP and BFS."""
 import os
 import subprocess
 import sys
@@ -68,6 +70,11 @@ for pack in (1, 0):
             for i, ref in enumerate(brefs):
                 assert np.array_equal(B.get(i), ref), (pack, drive, delta, i)
             n_solves += 1
+        for hold in ((1, 5) if pack else ()):
+            B.solve(npa.Sched(level=2, log2_t_lo=4, log2_t_hi=8, drive=drive, pack_hold=hold))
+            for i, ref in enumerate(brefs):
+                assert np.array_equal(B.get(i), ref), ("hold", hold, drive, i)
+            n_solves += 1
     B.free()
 E.free(); grid.free(); G.free(); ctx.close()
 print("bounds-checked solves exact:", n_solves, flush=True)
@@ -81,3 +88,58 @@ def test_bounds_checked_build_runs_clean():
                        stderr=subprocess.STDOUT, text=True, timeout=280)
     assert r.returncode == 0, r.stdout[-3000:]
     assert "bounds-checked solves exact:" in r.stdout, r.stdout[-3000:]
+
+
+CHILD_FULL = r'''
+import importlib.util, os, sys, types
+from concurrent.futures import ThreadPoolExecutor
+import numpy as np
+ROOT = sys.argv[1]
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+from conftest import load_pkg
+import oracle as orc
+npa = load_pkg()
+assert b"bounds-checked" in npa.load().np_version()
+spec = importlib.util.spec_from_file_location("np_bench", os.path.join(ROOT, "bench.py"))
+bench = importlib.util.module_from_spec(spec)
+spec.loader.exec_module(bench)
+ctx = npa.Context(0)
+W = bench.build_workload(npa, ctx, "c3", types.SimpleNamespace(scale=24, sources=16))
+G = W.graphs[0]
+srcs = [s for _, s in W.inst]
+host = G.download()
+B = npa.Batch(ctx, [G] * len(srcs), srcs)
+first = None
+for sch in (npa.Sched(level=2, log2_t_lo=4, log2_t_hi=10, drive=npa.DRIVE_GRAPH, pack_hold=11),
+            npa.Sched(level=2, log2_t_lo=4, log2_t_hi=10, drive=npa.DRIVE_PERSISTENT, pack_hold=10)):
+    B.solve(sch)
+    out = [B.get(i) for i in range(len(srcs))]
+    if first is None:
+        for s, d in zip(srcs, out):
+            code, bad = orc.check_sssp(*host, s, d)
+            assert code == 0, (s, orc.CHECK_CODES[code], bad)
+        first = out
+    else:
+        assert all(np.array_equal(a, b) for a, b in zip(first, out)), sch
+B.free()
+Bu = npa.Batch(ctx, [G] * len(srcs), srcs, unit=True)
+Bu.solve(npa.Sched(level=2, log2_t_lo=4, log2_t_hi=12, drive=npa.DRIVE_GRAPH, pack_hold=7))
+with ThreadPoolExecutor(4) as ex:
+    refs = list(ex.map(lambda s: orc.bfs_levels(host[0], host[1], s), srcs[:4]))
+for i, ref in enumerate(refs):
+    assert np.array_equal(Bu.get(i), ref), ("bfs", i)
+Bu.free()
+G.free()
+ctx.close()
+print("bounds-checked full-size solves exact", flush=True)
+'''
+
+
+@pytest.mark.timeout(300)
+def test_bounds_checked_build_full_size_packed_hold():
+    env = dict(os.environ, NP_SSSP_LIB="bounds")
+    r = subprocess.run([sys.executable, "-c", CHILD_FULL, ROOT], env=env, stdout=subprocess.PIPE,
+                       stderr=subprocess.STDOUT, text=True, timeout=280)
+    assert r.returncode == 0, r.stdout[-3000:]
+    assert "bounds-checked full-size solves exact" in r.stdout, r.stdout[-3000:]
